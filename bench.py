@@ -1,0 +1,253 @@
+"""Benchmark of the MI355X zero block codec on its headline workload.
+
+Workload (one GPU): 4 GiB of LZ-friendly synthetic bytes (seed 1) and 1 GiB of uniform random bytes (seed 2), level 5,
+blocks of 2^18 bytes, every buffer resident in HBM when a timed region starts. A step is one pass of the codec over the
+whole 4 GiB input: ``--steps`` encode passes and ``--steps`` decode passes are timed separately (HIP events on the
+codec's stream, bracketed by device synchronisation). ``value`` is the decode throughput in raw GB/s (10^9 B/s,
+the reference's convention: raw bytes / time); the encode throughput, the ratio and the random-data numbers sit beside
+it. ``roofline`` is the dominant decode kernel's algorithmic bytes (compressed + raw) per launch over its average
+launch duration against the 8 TB/s HBM peak. ``cpu_baseline`` is the reference's own CPU codec (oracle/_ref, compiled
+from the reference's headers) on a bounded sample of the same input, on this machine's cores.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--gib G]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "zero-compression_amd"))
+HBM_PEAK = 8.0e12
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(data, bl, depth, budget_s=12.0):
+    """The reference CpuCodec (oracle/_ref/libzeroref.so) on a sample of the input, all cores of this process."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Ref, ref_available
+    if not ref_available():
+        return None
+    ref = Ref()
+    threads = len(os.sched_getaffinity(0))
+    sample = data[: 256 << 20]
+    t0 = time.perf_counter()
+    blob, clen, meth = ref.cpu_compress(sample, bl, depth, threads)
+    tc = time.perf_counter() - t0
+    raw_len = np.array([min(1 << bl, sample.size - o) for o in range(0, sample.size, 1 << bl)], dtype=np.uint32)
+    reps, td = 0, 0.0
+    while reps < 3 or (td < budget_s / 3 and reps < 20):
+        t0 = time.perf_counter()
+        rc, back = ref.cpu_decompress(blob, clen, meth, raw_len, threads)
+        td += time.perf_counter() - t0
+        reps += 1
+        assert rc == 0
+    assert np.array_equal(back, sample)
+    return {
+        "value": round(sample.size * reps / td / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": "reference",
+        "sample": f"first {sample.size >> 20} MiB of the seed-1 LZ input, level 5 (chain depth {depth}), 2^{bl} blocks: "
+                  f"decode x{reps} (value), encode once",
+        "compress_GBps": round(sample.size / tc / 1e9, 4), "ratio": round(float(clen.sum()) / sample.size, 4),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--gib", type=float, default=4.0, help="size of the LZ input in GiB")
+    ap.add_argument("--level", type=int, default=5)
+    ap.add_argument("--block-log", type=int, default=18)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (blocks would shard across GPUs with one "
+              "all-gather of block lengths; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    import pyzero
+
+    bl, level = a.block_log, a.level
+    B = 1 << bl
+    n = int(a.gib * (1 << 30)) // B * B
+    nb = n // B
+    gpu = pyzero.Gpu(0)
+    props = gpu.props()
+    log(f"device {props['name']} {props['arch']} {props['cu_count']} CUs")
+    t0 = time.time()
+    data = pyzero.gen_lz(1, n)
+    rnd = pyzero.gen_random(2, 1 << 30)
+    log(f"generated {n >> 20} MiB LZ + 1024 MiB random in {time.time() - t0:.1f} s")
+
+    # ---- device buffers
+    d_in = gpu.alloc(n)
+    d_in.upload(data)
+    tab = pyzero.block_table([B] * nb, src=np.arange(nb, dtype=np.uint64) * B, dst=np.arange(nb, dtype=np.uint64) * B)
+    d_tab = gpu.alloc(tab.nbytes)
+    d_tab.upload(tab)
+    slabs = gpu.alloc(n)
+    lens = gpu.alloc(nb * 4)
+    batch = 4096  # blocks per encoder launch (1 GiB at 2^18)
+
+    def encode_pass(din, dtab, k_blocks, out_slabs, out_lens):
+        for b0 in range(0, k_blocks, batch):
+            k = min(batch, k_blocks - b0)
+            gpu.encode_blocks(din.ptr, dtab.addr(b0 * 32), k, bl, level, out_slabs.addr(b0 << bl),
+                              out_lens.addr(b0 * 4))
+
+    ev0, ev1 = gpu.event(), gpu.event()
+    # ---- encode
+    for _ in range(a.warmup):
+        encode_pass(d_in, d_tab, nb, slabs, lens)
+    gpu.sync()
+    gpu.record(ev0)
+    for _ in range(a.steps):
+        encode_pass(d_in, d_tab, nb, slabs, lens)
+    gpu.record(ev1)
+    gpu.sync()
+    enc_ms = gpu.elapsed_ms(ev0, ev1) / a.steps
+    log(f"encode {enc_ms:.2f} ms/step = {n / enc_ms / 1e6:.1f} GB/s")
+    gpu.set_profile(True)
+    encode_pass(d_in, d_tab, min(nb, batch), slabs, lens)
+    enc_stages = gpu.stage_times()
+    gpu.set_profile(False)
+    log("encode stages (ms, one 1 GiB launch): " + ", ".join(f"{k} {v:.2f}" for k, v in enc_stages))
+
+    # ---- the archive: payloads packed back to back (host side, untimed)
+    ln = lens.download(dtype=np.uint32).copy()
+    sl = slabs.download()
+    stored = ln == pyzero.STORED
+    clen = np.where(stored, B, ln).astype(np.uint64)
+    meth = np.where(stored, 0, 1).astype(np.uint32)
+    parts = [sl[b * B: b * B + int(ln[b])] if not stored[b] else data[b * B:(b + 1) * B] for b in range(nb)]
+    blob = np.concatenate(parts)
+    del sl, parts
+    comp_total = int(clen.sum())
+    ratio = comp_total / n
+    log(f"ratio {ratio:.4f} ({comp_total} / {n}), {int(stored.sum())} stored blocks")
+    slabs.free()
+    d_comp = gpu.alloc(blob.size)
+    d_comp.upload(blob)
+    dtab = pyzero.block_table([B] * nb, comp_lens=clen, methods=meth)
+    d_dtab = gpu.alloc(dtab.nbytes)
+    d_dtab.upload(dtab)
+    d_out = gpu.alloc(n)
+    d_err = gpu.alloc(nb * 4)
+
+    # ---- decode
+    for _ in range(a.warmup):
+        gpu.decode_blocks(d_comp.ptr, d_dtab.ptr, nb, bl, d_out.ptr, d_err.ptr)
+    gpu.sync()
+    gpu.set_profile(True)
+    stage_sum = {}
+    gpu.record(ev0)
+    for _ in range(a.steps):
+        gpu.decode_blocks(d_comp.ptr, d_dtab.ptr, nb, bl, d_out.ptr, d_err.ptr)
+        for k, v in gpu.stage_times():
+            stage_sum[k] = stage_sum.get(k, 0.0) + v
+    gpu.record(ev1)
+    gpu.sync()
+    gpu.set_profile(False)
+    dec_ms = gpu.elapsed_ms(ev0, ev1) / a.steps
+    stage_avg = {k: v / a.steps for k, v in stage_sum.items()}
+    log(f"decode {dec_ms:.3f} ms/step = {n / dec_ms / 1e6:.1f} GB/s; stages {stage_avg}")
+    # timed again without the per-stage events or syncs, for the headline
+    gpu.record(ev0)
+    for _ in range(a.steps):
+        gpu.decode_blocks(d_comp.ptr, d_dtab.ptr, nb, bl, d_out.ptr, d_err.ptr)
+    gpu.record(ev1)
+    gpu.sync()
+    dec_ms = gpu.elapsed_ms(ev0, ev1) / a.steps
+    err = d_err.download(dtype=np.uint32)
+    ok = bool((err == 0).all()) and np.array_equal(d_out.download(), data)
+    log(f"decode {dec_ms:.3f} ms/step = {n / dec_ms / 1e6:.1f} GB/s, round trip {'identical' if ok else 'MISMATCH'}")
+    dom = max(stage_avg, key=stage_avg.get)
+    alg = comp_total + n
+    roof = {"bound": "hbm", "kernel": f"k_dec_{dom}", "achieved": round(alg / (stage_avg[dom] * 1e-3) / 1e9, 1),
+            "peak": HBM_PEAK / 1e9, "unit": "GB/s"}
+    roof["frac"] = round(roof["achieved"] * 1e9 / HBM_PEAK, 4)
+    roof["traffic"] = None
+    roof["whole_decode_frac"] = round(alg / (dec_ms * 1e-3) / HBM_PEAK, 4)
+
+    # ---- random 1 GiB: every block stored
+    for buf in (d_comp, d_out):
+        buf.free()
+    nr = rnd.size // B
+    r_in = gpu.alloc(rnd.size)
+    r_in.upload(rnd)
+    r_slabs = gpu.alloc(rnd.size)
+    r_lens = gpu.alloc(nr * 4)
+    rtab = pyzero.block_table([B] * nr, src=np.arange(nr, dtype=np.uint64) * B, dst=np.arange(nr, dtype=np.uint64) * B)
+    r_tab = gpu.alloc(rtab.nbytes)
+    r_tab.upload(rtab)
+    encode_pass(r_in, r_tab, nr, r_slabs, r_lens)
+    gpu.sync()
+    gpu.record(ev0)
+    for _ in range(a.steps):
+        encode_pass(r_in, r_tab, nr, r_slabs, r_lens)
+    gpu.record(ev1)
+    gpu.sync()
+    r_enc_ms = gpu.elapsed_ms(ev0, ev1) / a.steps
+    rl = r_lens.download(dtype=np.uint32)
+    r_stored = int((rl == pyzero.STORED).sum())
+    rdt = pyzero.block_table([B] * nr, comp_lens=[B] * nr, methods=[0] * nr)
+    r_tab.upload(rdt)
+    r_out = gpu.alloc(rnd.size)
+    r_err = gpu.alloc(nr * 4)
+    gpu.decode_blocks(r_in.ptr, r_tab.ptr, nr, bl, r_out.ptr, r_err.ptr)
+    gpu.sync()
+    gpu.record(ev0)
+    for _ in range(a.steps):
+        gpu.decode_blocks(r_in.ptr, r_tab.ptr, nr, bl, r_out.ptr, r_err.ptr)
+    gpu.record(ev1)
+    gpu.sync()
+    r_dec_ms = gpu.elapsed_ms(ev0, ev1) / a.steps
+    r_ok = np.array_equal(r_out.download(), rnd)
+    log(f"random: encode {rnd.size / r_enc_ms / 1e6:.1f} GB/s ({r_stored}/{nr} stored), "
+        f"decode {rnd.size / r_dec_ms / 1e6:.1f} GB/s, {'identical' if r_ok else 'MISMATCH'}")
+
+    cpu = None if a.no_cpu else cpu_baseline(data, bl, {1: 8, 9: 256}.get(level, 32))
+    if cpu:
+        log(f"cpu baseline: {cpu}")
+    line = {
+        "metric": "compress/decompress GB/s + ratio at L5/256KiB; p99 latency at fixed load",
+        "value": round(n / (dec_ms * 1e-3) / 1e9, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(dec_ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (seeded generators; no dataset)",
+        "config": {"workload": f"plain GPU drop-in: {n >> 30} GiB LZ-friendly seed 1 + 1 GiB random seed 2, "
+                               f"level {level}, 2^{bl} blocks, device-resident; step = decode of the whole LZ input",
+                   "level": level, "block_log": bl, "input_bytes": n, "blocks": nb},
+        "decompress_GBps": round(n / (dec_ms * 1e-3) / 1e9, 3),
+        "compress_GBps": round(n / (enc_ms * 1e-3) / 1e9, 3),
+        "compress_ms_per_step": round(enc_ms, 3),
+        "ratio": round(ratio, 5),
+        "roundtrip_identical": bool(ok and r_ok),
+        "random": {"compress_GBps": round(rnd.size / (r_enc_ms * 1e-3) / 1e9, 3),
+                   "decompress_GBps": round(rnd.size / (r_dec_ms * 1e-3) / 1e9, 3),
+                   "stored_blocks": r_stored, "blocks": nr},
+        "decode_stage_ms": {k: round(v, 4) for k, v in stage_avg.items()},
+        "encode_stage_ms_per_GiB": {k: round(v, 3) for k, v in enc_stages},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "device": props["name"],
+    }
+    print(json.dumps(line), flush=True)
+    gpu.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,128 @@
+/* zero_hip.h: the C ABI of the MI355X (gfx950) zero block codec, libzero_hip.so.
+ *
+ * It replaces the reference's GPU runtime layer (src/gpu.h, src/ze_min.h: device, buffers, kernel launches) and the
+ * kernel sequences that src/gpu_codec.h drives, with ahead-of-time compiled HIP kernels. Plain pointers and sizes
+ * only; every function returns ZGPU_OK (0) or a negative ZGPU_ERR_* code, with a message from zgpu_last_error().
+ * A context is used by one host thread at a time (as the reference's GpuCodec is).
+ *
+ * Formats (bit-exact with the reference): the ZB2 block payload (src/codec2.h:16-17,140-151) with its rANS stream
+ * sections (src/codec.h:4-6,96-147), and the CRC-32 of src/crc32.h.
+ */
+#ifndef ZERO_HIP_H
+#define ZERO_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ZGPU_OK 0
+#define ZGPU_ERR_DEVICE (-1)  /* the HIP runtime failed (launch failure, lost device): src/gpu.h:17-27 gpu::Error */
+#define ZGPU_ERR_ARG (-2)     /* an argument is invalid (block size, level, sizes) */
+#define ZGPU_ERR_DAMAGED (-3) /* at least one block is damaged (per-block bits in err[]): src/codec.h:29-31 zb::Error */
+#define ZGPU_ERR_NOMEM (-4)   /* device memory exhausted */
+
+/* per-block error bits written to err[] by the decoder */
+#define ZGPU_E_HDR 0x01u
+#define ZGPU_E_COUNTS 0x02u
+#define ZGPU_E_STREAM 0x04u
+#define ZGPU_E_TABLE 0x08u
+#define ZGPU_E_LANE 0x10u
+#define ZGPU_E_TOKEN 0x20u
+#define ZGPU_E_GROUP 0x40u
+#define ZGPU_E_METHOD 0x80u
+
+#define ZGPU_STORED 0xFFFFFFFFu /* out_len of a block whose payload would not be smaller: write it stored */
+
+typedef struct zgpu zgpu_t;
+typedef void* zgpu_stream_t; /* a hipStream_t; NULL is the context's own stream */
+
+/* One entry per block of a launch (32 bytes, device memory for the zgpu_*_blocks calls). Encoding reads raw_len bytes
+ * at in + src. Decoding reads comp_len bytes at comp + src (method 0 stored, 1 ZB2) and writes raw_len bytes at
+ * out + dst. Mirrors zero::BlockIn (src/archive.h:50-55) plus where the bytes live. */
+typedef struct {
+  uint64_t src;
+  uint64_t dst;
+  uint32_t raw_len;
+  uint32_t comp_len;
+  uint32_t method;
+  uint32_t pad;
+} zgpu_block_t;
+
+typedef struct {
+  char name[128];      /* marketing name */
+  char arch[64];       /* gcnArchName, e.g. gfx950 */
+  int cu_count;
+  int clock_khz;
+  uint64_t hbm_bytes;
+  uint32_t lds_per_block; /* LDS a workgroup may use */
+  uint32_t max_group;
+  int integrated;
+} zgpu_props_t;
+
+/* ---- device (src/gpu.h:124-257 gpu::Device::open / sync / properties) */
+int zgpu_open(int ordinal, zgpu_t** out);
+void zgpu_close(zgpu_t* g);
+int zgpu_props(zgpu_t* g, zgpu_props_t* out);
+const char* zgpu_last_error(zgpu_t* g);
+int zgpu_sync(zgpu_t* g);
+
+/* ---- memory (src/gpu.h:200-252 gpu::Buffer alloc / alloc_host / write / read / zero) */
+int zgpu_alloc(zgpu_t* g, size_t bytes, void** dptr);
+int zgpu_free(zgpu_t* g, void* dptr);
+int zgpu_alloc_host(zgpu_t* g, size_t bytes, void** hptr); /* pinned */
+int zgpu_free_host(zgpu_t* g, void* hptr);
+int zgpu_upload(zgpu_t* g, void* dptr, const void* host, size_t bytes);   /* synchronous */
+int zgpu_download(zgpu_t* g, void* host, const void* dptr, size_t bytes); /* synchronous */
+int zgpu_fill(zgpu_t* g, void* dptr, uint8_t value, size_t bytes, zgpu_stream_t s); /* a kernel, graph-safe */
+
+/* ---- streams and events (timing on the stream the kernels run on) */
+int zgpu_stream_create(zgpu_t* g, zgpu_stream_t* out);
+int zgpu_stream_destroy(zgpu_t* g, zgpu_stream_t s);
+int zgpu_stream_sync(zgpu_t* g, zgpu_stream_t s);
+int zgpu_event_create(zgpu_t* g, void** ev);
+int zgpu_event_destroy(zgpu_t* g, void* ev);
+int zgpu_event_record(zgpu_t* g, void* ev, zgpu_stream_t s);
+int zgpu_event_elapsed_ms(zgpu_t* g, void* start, void* stop, float* ms);
+
+/* ---- the codec on device-resident buffers (src/gpu_codec.h:153-229 compress, :253-315 decompress)
+ * block_log 16..20; level 1..9 (src/gpu_codec.h:66-70: 1-2 two neighbours / 16 bytes compared, 3-6 four / 16,
+ * 7-9 eight / 32). Scratch is the context's, grown on demand. */
+
+/* Encodes nb blocks (each raw_len <= 2^block_log). Block b's payload goes to out + (b << block_log) and its size to
+ * out_len[b], or ZGPU_STORED when it would not be smaller than the block. */
+int zgpu_encode_blocks(zgpu_t* g, const uint8_t* d_in, const zgpu_block_t* d_blocks, uint32_t nb, uint32_t block_log,
+                       int level, uint8_t* d_out, uint32_t* d_out_len, zgpu_stream_t s);
+/* Decodes nb blocks into out; err[b] receives ZGPU_E_* bits (0 = good). Asynchronous: read err after syncing. */
+int zgpu_decode_blocks(zgpu_t* g, const uint8_t* d_comp, const zgpu_block_t* d_blocks, uint32_t nb, uint32_t block_log,
+                       uint8_t* d_out, uint32_t* d_err, zgpu_stream_t s);
+/* crc[b] = CRC-32 of block b's raw bytes, at data + dst (use_dst != 0) or data + src (src/archive.h:376,618) */
+int zgpu_crc32_blocks(zgpu_t* g, const uint8_t* d_data, const zgpu_block_t* d_blocks, uint32_t nb, int use_dst,
+                      uint32_t* d_crc, zgpu_stream_t s);
+/* device scratch the context holds for nb blocks of 2^block_log */
+size_t zgpu_encode_scratch_bytes(uint32_t nb, uint32_t block_log);
+size_t zgpu_decode_scratch_bytes(uint32_t nb, uint32_t block_log);
+
+/* ---- the codec on host buffers: zero::Codec::compress / decompress (src/archive.h:56-69) as GpuCodec does them
+ * (upload, kernels, download; synchronous). */
+/* n bytes = whole blocks, the last may be shorter. out: (nb << block_log) bytes, block b's payload at b << block_log;
+ * out_len[b] = payload size, or ZGPU_STORED. */
+int zgpu_compress(zgpu_t* g, const uint8_t* in, uint64_t n, uint32_t block_log, int level, uint8_t* out,
+                  uint32_t* out_len);
+/* blocks back to back in comp (comp_len[b] bytes each), their raw bytes back to back into out. err[b] as above;
+ * returns ZGPU_ERR_DAMAGED when any block is damaged. */
+int zgpu_decompress(zgpu_t* g, const uint8_t* comp, const uint32_t* comp_len, const uint8_t* method,
+                    const uint32_t* raw_len, uint32_t nb, uint32_t block_log, uint8_t* out, uint32_t* err);
+/* crc[b] of the nb blocks of 2^block_log in data[0..n) (host buffer) */
+int zgpu_crc32(zgpu_t* g, const uint8_t* data, uint64_t n, uint32_t block_log, uint32_t* crc);
+
+/* ---- stage timing (ZERO_PROFILE's per-stage split, src/gpu_codec.h:32-49): when on, events are recorded between
+ * the kernels of every encode/decode call on its stream; zgpu_stage_times reads the last call's (it syncs). */
+int zgpu_set_profile(zgpu_t* g, int on);
+int zgpu_stage_times(zgpu_t* g, const char** names, float* ms, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

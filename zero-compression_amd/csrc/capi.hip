@@ -1,0 +1,487 @@
+// The C ABI of include/zero_hip.h: device context, memory, streams, and the launch sequences of the encoder and
+// decoder (the roles of src/gpu.h and of GpuCodec::compress / decompress in src/gpu_codec.h:153-315).
+// One translation unit holds every kernel, so libzero_hip.so is a single code object for gfx950.
+#include "crc.hip"
+#include "decode.hip"
+#include "encode.hip"
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/zero_hip.h"
+
+using namespace zk;
+
+namespace {
+
+// a device buffer that grows on demand
+struct DBuf {
+  void* p = nullptr;
+  size_t n = 0;
+};
+
+struct Stage {
+  const char* name;
+  hipEvent_t ev;
+};
+
+}  // namespace
+
+struct zgpu {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  zgpu_props_t props{};
+  uint32_t* xpow = nullptr;  // x^(8 m 256^j) mod P, 3 x 256 (CRC combine)
+  DBuf enc, dec, aux;  // encoder scratch, decoder scratch, staging of the host-buffer forms
+  bool profile = false;
+  std::vector<Stage> stages;
+  std::vector<hipEvent_t> pool;
+  size_t used_events = 0;
+};
+
+static int fail(zgpu_t* g, int code, const std::string& msg) {
+  if (g) g->err = msg;
+  return code;
+}
+static int hip_fail(zgpu_t* g, hipError_t e, const char* what) {
+  const int code = e == hipErrorOutOfMemory ? ZGPU_ERR_NOMEM : ZGPU_ERR_DEVICE;
+  // device loss and launch failures read as the reference's "the GPU was reset" (src/gpu.h:20-27)
+  std::string m = std::string(what) + ": " + hipGetErrorString(e);
+  if (e == hipErrorLaunchFailure || e == hipErrorIllegalAddress || e == hipErrorECCNotCorrectable)
+    m += " (the GPU was reset)";
+  return fail(g, code, m);
+}
+#define HIPCHK(g, call)                                  \
+  do {                                                   \
+    const hipError_t e_ = (call);                        \
+    if (e_ != hipSuccess) return hip_fail(g, e_, #call); \
+  } while (0)
+
+static int ensure(zgpu_t* g, DBuf& b, size_t bytes) {
+  if (b.n >= bytes) return ZGPU_OK;
+  if (b.p) HIPCHK(g, hipFree(b.p));
+  b.p = nullptr;
+  b.n = 0;
+  HIPCHK(g, hipMalloc(&b.p, bytes));
+  b.n = bytes;
+  return ZGPU_OK;
+}
+
+static void stage_begin(zgpu_t* g, hipStream_t s) {
+  g->stages.clear();
+  g->used_events = 0;
+  if (!g->profile) return;
+  if (g->pool.empty()) {
+    for (int i = 0; i < 32; ++i) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return;
+      g->pool.push_back(e);
+    }
+  }
+  g->stages.push_back({"start", g->pool[g->used_events++]});
+  (void)hipEventRecord(g->stages.back().ev, s);
+}
+static void stage(zgpu_t* g, hipStream_t s, const char* name) {
+  if (!g->profile || g->used_events >= g->pool.size()) return;
+  g->stages.push_back({name, g->pool[g->used_events++]});
+  (void)hipEventRecord(g->stages.back().ev, s);
+}
+
+// ---------------------------------------------------------------- small kernels
+__global__ void k_fill(uint8_t* p, uint8_t v, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = i; k < n; k += stride) p[k] = v;
+}
+
+// ---------------------------------------------------------------- scratch layouts
+namespace {
+inline size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct EncLayout {
+  size_t srt, mtch, bstart, qmid, qlarge, qcnt, mark, lit, LB, TM, TD, tok, sym, grp, ext, offw, cnt, lanes, freq,
+      lanelen, sinfo, total;
+  uint32_t qcap;
+};
+EncLayout enc_layout(uint32_t nb, uint32_t blog) {
+  const Geo G = geo_of(blog);
+  const size_t N = (size_t)nb << blog;
+  const size_t HB = (size_t)1 << hbits_of(blog);
+  EncLayout L;
+  size_t o = 0;
+  L.qcap = (uint32_t)((size_t)nb * (G.B / 17u + 1u));
+  L.srt = o; o += al(N * 4);                // sorted bucket entries, then the lazy matches
+  L.mtch = o; o += al(N * 4);               // radix temporary, then the matches
+  L.bstart = o; o += al((size_t)nb * (HB + 1) * 4);
+  L.qmid = o; o += al((size_t)L.qcap * 8);
+  L.qlarge = o; o += al((size_t)L.qcap * 8);
+  L.qcnt = o; o += al(16);
+  L.mark = o; o += al(N / 8);
+  L.lit = o; o += al(N);
+  L.LB = o; o += al((size_t)nb * G.TCAP * 4);
+  L.TM = o; o += al((size_t)nb * G.TCAP * 4);
+  L.TD = o; o += al((size_t)nb * G.TCAP * 4);
+  L.tok = o; o += al((size_t)nb * G.TCAP);
+  L.sym = o; o += al((size_t)nb * G.TCAP);
+  L.grp = o; o += al((size_t)nb * G.TCAP);
+  L.ext = o; o += al((size_t)nb * G.ECAP);
+  L.offw = o; o += al((size_t)nb * (G.OCAP / 4 + 1) * 4);
+  L.cnt = o; o += al((size_t)nb * 8 * 4);
+  L.lanes = o; o += al((size_t)nb * lanes_per_block(G));
+  L.freq = o; o += al((size_t)nb * 5 * 256 * 2);
+  L.lanelen = o; o += al((size_t)nb * 5 * 64 * 4);
+  L.sinfo = o; o += al((size_t)nb * 5 * 4 * 4);
+  L.total = o;
+  return L;
+}
+
+struct DecLayout {
+  size_t lit, tok, ext, sym, grp, total;
+};
+DecLayout dec_layout(uint32_t nb, uint32_t blog) {
+  const Geo G = geo_of(blog);
+  DecLayout L;
+  size_t o = 0;
+  L.lit = o; o += al((size_t)nb << blog);
+  L.tok = o; o += al((size_t)nb * G.TCAP);
+  L.ext = o; o += al((size_t)nb * G.ECAP);
+  L.sym = o; o += al((size_t)nb * G.TCAP);
+  L.grp = o; o += al((size_t)nb * G.TCAP);
+  L.total = o;
+  return L;
+}
+
+uint32_t host_mulmod(uint32_t a, uint32_t b) {
+  uint32_t r = 0;
+  for (int i = 31; i >= 0; --i) {
+    if ((a >> i) & 1u) r ^= b;
+    b = (b >> 1) ^ ((b & 1u) ? 0xEDB88320u : 0u);
+  }
+  return r;
+}
+}  // namespace
+
+extern "C" {
+
+size_t zgpu_encode_scratch_bytes(uint32_t nb, uint32_t block_log) { return enc_layout(nb, block_log).total; }
+size_t zgpu_decode_scratch_bytes(uint32_t nb, uint32_t block_log) { return dec_layout(nb, block_log).total; }
+
+int zgpu_open(int ordinal, zgpu_t** out) {
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(nullptr, ZGPU_ERR_DEVICE, "no GPU");
+  if (ordinal < 0 || ordinal >= n) return ZGPU_ERR_ARG;
+  zgpu_t* g = new zgpu();
+  g->dev = ordinal;
+  hipError_t e = hipSetDevice(ordinal);
+  hipDeviceProp_t p;
+  if (e == hipSuccess) e = hipGetDeviceProperties(&p, ordinal);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete g; return ZGPU_ERR_DEVICE; }
+  snprintf(g->props.name, sizeof g->props.name, "%s", p.name);
+  snprintf(g->props.arch, sizeof g->props.arch, "%s", p.gcnArchName);
+  g->props.cu_count = p.multiProcessorCount;
+  g->props.clock_khz = p.clockRate;
+  g->props.hbm_bytes = p.totalGlobalMem;
+  g->props.lds_per_block = (uint32_t)p.sharedMemPerBlock;
+  g->props.max_group = (uint32_t)p.maxThreadsPerBlock;
+  g->props.integrated = p.integrated;
+  // CRC combine powers: xp[j*256 + m] = x^(8 m 256^j) mod P
+  std::vector<uint32_t> xp(768);
+  uint32_t step = 1u << 23;  // x^8 (bit 31 is x^0)
+  for (int j = 0; j < 3; ++j) {
+    uint32_t acc = 1u << 31;
+    for (int m = 0; m < 256; ++m) { xp[j * 256 + m] = acc; acc = host_mulmod(acc, step); }
+    step = acc;  // x^(8 * 256^(j+1))
+  }
+  if (hipMalloc(&g->xpow, 768 * 4) != hipSuccess ||
+      hipMemcpy(g->xpow, xp.data(), 768 * 4, hipMemcpyHostToDevice) != hipSuccess) {
+    delete g;
+    return ZGPU_ERR_DEVICE;
+  }
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_enc_hash), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            160 * 1024);
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_enc_parse), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            160 * 1024);
+  if (const char* pe = getenv("ZERO_PROFILE")) g->profile = atoi(pe) != 0;
+  *out = g;
+  return ZGPU_OK;
+}
+
+void zgpu_close(zgpu_t* g) {
+  if (!g) return;
+  (void)hipSetDevice(g->dev);
+  (void)hipStreamSynchronize(g->stream);
+  for (DBuf* b : {&g->enc, &g->dec, &g->aux})
+    if (b->p) (void)hipFree(b->p);
+  for (hipEvent_t e : g->pool) (void)hipEventDestroy(e);
+  if (g->xpow) (void)hipFree(g->xpow);
+  (void)hipStreamDestroy(g->stream);
+  delete g;
+}
+
+int zgpu_props(zgpu_t* g, zgpu_props_t* out) {
+  *out = g->props;
+  return ZGPU_OK;
+}
+const char* zgpu_last_error(zgpu_t* g) { return g ? g->err.c_str() : "no context"; }
+int zgpu_sync(zgpu_t* g) {
+  HIPCHK(g, hipStreamSynchronize(g->stream));
+  HIPCHK(g, hipDeviceSynchronize());
+  return ZGPU_OK;
+}
+
+int zgpu_alloc(zgpu_t* g, size_t bytes, void** dptr) { HIPCHK(g, hipMalloc(dptr, bytes ? bytes : 1)); return ZGPU_OK; }
+int zgpu_free(zgpu_t* g, void* dptr) { HIPCHK(g, hipFree(dptr)); return ZGPU_OK; }
+int zgpu_alloc_host(zgpu_t* g, size_t bytes, void** hptr) {
+  HIPCHK(g, hipHostMalloc(hptr, bytes ? bytes : 1, hipHostMallocDefault));
+  return ZGPU_OK;
+}
+int zgpu_free_host(zgpu_t* g, void* hptr) { HIPCHK(g, hipHostFree(hptr)); return ZGPU_OK; }
+int zgpu_upload(zgpu_t* g, void* dptr, const void* host, size_t bytes) {
+  if (!bytes) return ZGPU_OK;
+  HIPCHK(g, hipMemcpyAsync(dptr, host, bytes, hipMemcpyHostToDevice, g->stream));
+  HIPCHK(g, hipStreamSynchronize(g->stream));
+  return ZGPU_OK;
+}
+int zgpu_download(zgpu_t* g, void* host, const void* dptr, size_t bytes) {
+  if (!bytes) return ZGPU_OK;
+  HIPCHK(g, hipMemcpyAsync(host, dptr, bytes, hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(g, hipStreamSynchronize(g->stream));
+  return ZGPU_OK;
+}
+int zgpu_fill(zgpu_t* g, void* dptr, uint8_t value, size_t bytes, zgpu_stream_t s) {
+  if (!bytes) return ZGPU_OK;
+  hipStream_t st = s ? (hipStream_t)s : g->stream;
+  const size_t groups = std::min<size_t>((bytes + 255) / 256, 4096);
+  k_fill<<<(unsigned)groups, 256, 0, st>>>((uint8_t*)dptr, value, bytes);
+  HIPCHK(g, hipGetLastError());
+  return ZGPU_OK;
+}
+
+int zgpu_stream_create(zgpu_t* g, zgpu_stream_t* out) {
+  hipStream_t s;
+  HIPCHK(g, hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *out = s;
+  return ZGPU_OK;
+}
+int zgpu_stream_destroy(zgpu_t* g, zgpu_stream_t s) { HIPCHK(g, hipStreamDestroy((hipStream_t)s)); return ZGPU_OK; }
+int zgpu_stream_sync(zgpu_t* g, zgpu_stream_t s) {
+  HIPCHK(g, hipStreamSynchronize(s ? (hipStream_t)s : g->stream));
+  return ZGPU_OK;
+}
+int zgpu_event_create(zgpu_t* g, void** ev) {
+  hipEvent_t e;
+  HIPCHK(g, hipEventCreate(&e));
+  *ev = e;
+  return ZGPU_OK;
+}
+int zgpu_event_destroy(zgpu_t* g, void* ev) { HIPCHK(g, hipEventDestroy((hipEvent_t)ev)); return ZGPU_OK; }
+int zgpu_event_record(zgpu_t* g, void* ev, zgpu_stream_t s) {
+  HIPCHK(g, hipEventRecord((hipEvent_t)ev, s ? (hipStream_t)s : g->stream));
+  return ZGPU_OK;
+}
+int zgpu_event_elapsed_ms(zgpu_t* g, void* a, void* b, float* ms) {
+  HIPCHK(g, hipEventSynchronize((hipEvent_t)b));
+  HIPCHK(g, hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b));
+  return ZGPU_OK;
+}
+
+int zgpu_set_profile(zgpu_t* g, int on) { g->profile = on != 0; return ZGPU_OK; }
+int zgpu_stage_times(zgpu_t* g, const char** names, float* ms, int cap) {
+  int k = 0;
+  for (size_t i = 1; i < g->stages.size() && k < cap; ++i, ++k) {
+    HIPCHK(g, hipEventSynchronize(g->stages[i].ev));
+    HIPCHK(g, hipEventElapsedTime(&ms[k], g->stages[i - 1].ev, g->stages[i].ev));
+    names[k] = g->stages[i].name;
+  }
+  return k;
+}
+
+// ---------------------------------------------------------------- encode
+int zgpu_encode_blocks(zgpu_t* g, const uint8_t* d_in, const zgpu_block_t* d_blocks, uint32_t nb, uint32_t blog,
+                       int level, uint8_t* d_out, uint32_t* d_out_len, zgpu_stream_t s) {
+  if (blog < 16 || blog > 20 || level < 1 || level > 9) return fail(g, ZGPU_ERR_ARG, "block_log 16..20, level 1..9");
+  if (nb == 0) return ZGPU_OK;
+  hipStream_t st = s ? (hipStream_t)s : g->stream;
+  const EncLayout L = enc_layout(nb, blog);
+  if (int rc = ensure(g, g->enc, L.total)) return rc;
+  uint8_t* base = (uint8_t*)g->enc.p;
+  const Block* blocks = (const Block*)d_blocks;
+  const Geo G = geo_of(blog);
+  const uint32_t ncand = level <= 2 ? 2 : level <= 6 ? 4 : 8;
+  const uint32_t lencap = level <= 6 ? 16 : 32;
+  const uint32_t HB = 1u << hbits_of(blog);
+  uint32_t* srt = (uint32_t*)(base + L.srt);
+  uint32_t* mt = (uint32_t*)(base + L.mtch);
+  uint32_t* bstart = (uint32_t*)(base + L.bstart);
+  SortQueues q{(uint32_t*)(base + L.qmid), (uint32_t*)(base + L.qlarge), (uint32_t*)(base + L.qcnt), L.qcap};
+  TokBufs T{base + L.lit,  (uint32_t*)(base + L.LB), (uint32_t*)(base + L.TM), (uint32_t*)(base + L.TD),
+            base + L.tok,  base + L.sym,              base + L.grp,              base + L.ext,
+            (uint32_t*)(base + L.offw), (uint32_t*)(base + L.cnt)};
+  RansBufs R{base + L.lanes, (uint16_t*)(base + L.freq), (uint32_t*)(base + L.lanelen), (uint32_t*)(base + L.sinfo)};
+  const uint32_t posgroups = nb << (blog - 8);
+  stage_begin(g, st);
+  k_fill<<<1, 64, 0, st>>>((uint8_t*)q.counts, 0, 16);
+  k_enc_hash<<<nb, 1024, (HB + 1024) * 4, st>>>(blocks, d_in, blog, srt, bstart);
+  stage(g, st, "hash");
+  k_enc_sort_s<<<(unsigned)(((size_t)nb * HB + 255) / 256), 256, 0, st>>>(blocks, nb, blog, srt, bstart, q);
+  k_enc_sort_m<<<std::min<uint32_t>(2048, L.qcap), 256, 0, st>>>(blog, srt, bstart, q);
+  k_enc_sort_l<<<std::min<uint32_t>(1024, L.qcap), 1024, 0, st>>>(blog, srt, mt, bstart, q);
+  stage(g, st, "sort");
+  k_enc_match<<<posgroups, 256, 0, st>>>(blocks, d_in, blog, ncand, lencap, srt, bstart, mt);
+  stage(g, st, "match");
+  k_enc_lazy<<<posgroups, 256, 0, st>>>(blocks, blog, mt, srt);
+  stage(g, st, "lazy");
+  const size_t parse_lds = ((size_t)G.B / 32 + 256 + G.B / 256) * 4;
+  k_enc_parse<<<nb, 256, parse_lds, st>>>(blocks, d_in, blog, lencap, srt, (uint32_t*)(base + L.mark));
+  stage(g, st, "parse");
+  k_enc_tokens<<<nb, 64, 0, st>>>(blocks, d_in, blog, srt, (uint32_t*)(base + L.mark), T);
+  stage(g, st, "tokens");
+  k_enc_rans<<<nb * 5, 64, 0, st>>>(blog, T, R);
+  stage(g, st, "rans");
+  k_enc_pack<<<nb, 256, 0, st>>>(blocks, blog, T, R, d_out, d_out_len);
+  stage(g, st, "pack");
+  HIPCHK(g, hipGetLastError());
+  return ZGPU_OK;
+}
+
+// ---------------------------------------------------------------- decode
+int zgpu_decode_blocks(zgpu_t* g, const uint8_t* d_comp, const zgpu_block_t* d_blocks, uint32_t nb, uint32_t blog,
+                       uint8_t* d_out, uint32_t* d_err, zgpu_stream_t s) {
+  if (blog < 16 || blog > 20) return fail(g, ZGPU_ERR_ARG, "block_log 16..20");
+  if (nb == 0) return ZGPU_OK;
+  hipStream_t st = s ? (hipStream_t)s : g->stream;
+  const DecLayout L = dec_layout(nb, blog);
+  if (int rc = ensure(g, g->dec, L.total)) return rc;
+  uint8_t* base = (uint8_t*)g->dec.p;
+  const Block* blocks = (const Block*)d_blocks;
+  stage_begin(g, st);
+  k_fill<<<(unsigned)std::min<size_t>((nb * 4 + 255) / 256, 1024), 256, 0, st>>>((uint8_t*)d_err, 0, (size_t)nb * 4);
+  k_dec_streams<<<nb * 5, 64, 0, st>>>(blocks, d_comp, blog, base + L.lit, base + L.tok, base + L.ext, base + L.sym,
+                                       base + L.grp, d_err);
+  stage(g, st, "streams");
+  k_dec_lz<<<nb, 64, 0, st>>>(blocks, d_comp, blog, base + L.lit, base + L.tok, base + L.ext, base + L.sym,
+                              base + L.grp, d_out, d_err);
+  stage(g, st, "lz");
+  HIPCHK(g, hipGetLastError());
+  return ZGPU_OK;
+}
+
+int zgpu_crc32_blocks(zgpu_t* g, const uint8_t* d_data, const zgpu_block_t* d_blocks, uint32_t nb, int use_dst,
+                      uint32_t* d_crc, zgpu_stream_t s) {
+  if (nb == 0) return ZGPU_OK;
+  hipStream_t st = s ? (hipStream_t)s : g->stream;
+  k_crc32<<<(nb + 3) / 4, 256, 0, st>>>((const Block*)d_blocks, nb, d_data, use_dst, g->xpow, d_crc);
+  HIPCHK(g, hipGetLastError());
+  return ZGPU_OK;
+}
+
+// ---------------------------------------------------------------- host-buffer forms
+int zgpu_compress(zgpu_t* g, const uint8_t* in, uint64_t n, uint32_t blog, int level, uint8_t* out,
+                  uint32_t* out_len) {
+  if (blog < 16 || blog > 20 || level < 1 || level > 9) return fail(g, ZGPU_ERR_ARG, "block_log 16..20, level 1..9");
+  const uint64_t B = 1ull << blog;
+  const uint32_t nb = (uint32_t)((n + B - 1) / B);
+  if (nb == 0) return ZGPU_OK;
+  std::vector<zgpu_block_t> tab(nb);
+  for (uint32_t b = 0; b < nb; ++b) {
+    tab[b].src = (uint64_t)b * B;
+    tab[b].dst = (uint64_t)b * B;
+    tab[b].raw_len = (uint32_t)std::min<uint64_t>(B, n - (uint64_t)b * B);
+    tab[b].comp_len = 0;
+    tab[b].method = 0;
+    tab[b].pad = 0;
+  }
+  const size_t outb = (size_t)nb * B;
+  if (int rc = ensure(g, g->aux, al(n) + al(outb) + al((size_t)nb * 4) + al((size_t)nb * 32))) return rc;
+  uint8_t* d_in = (uint8_t*)g->aux.p;
+  uint8_t* d_out = d_in + al(n);
+  uint32_t* d_len = (uint32_t*)(d_out + al(outb));
+  zgpu_block_t* d_tab = (zgpu_block_t*)((uint8_t*)d_len + al((size_t)nb * 4));
+  HIPCHK(g, hipMemcpyAsync(d_in, in, n, hipMemcpyHostToDevice, g->stream));
+  HIPCHK(g, hipMemcpyAsync(d_tab, tab.data(), (size_t)nb * 32, hipMemcpyHostToDevice, g->stream));
+  if (int rc = zgpu_encode_blocks(g, d_in, d_tab, nb, blog, level, d_out, d_len, nullptr)) return rc;
+  HIPCHK(g, hipMemcpyAsync(out_len, d_len, (size_t)nb * 4, hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(g, hipStreamSynchronize(g->stream));
+  // the payloads: one read of the slabs from the first to the last compressed block (src/gpu_codec.h:210-218)
+  uint32_t first = nb, last = 0;
+  for (uint32_t b = 0; b < nb; ++b)
+    if (out_len[b] != ZGPU_STORED) { if (b < first) first = b; last = b; }
+  if (first < nb) {
+    HIPCHK(g, hipMemcpyAsync(out + (size_t)first * B, d_out + (size_t)first * B, (size_t)(last - first + 1) * B,
+                             hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(g, hipStreamSynchronize(g->stream));
+  }
+  for (uint32_t b = 0; b < nb; ++b)
+    if (out_len[b] != ZGPU_STORED && out_len[b] >= tab[b].raw_len)
+      return fail(g, ZGPU_ERR_DEVICE, "GPU encoder produced an impossible block size");
+  return ZGPU_OK;
+}
+
+int zgpu_decompress(zgpu_t* g, const uint8_t* comp, const uint32_t* comp_len, const uint8_t* method,
+                    const uint32_t* raw_len, uint32_t nb, uint32_t blog, uint8_t* out, uint32_t* err) {
+  if (blog < 16 || blog > 20) return fail(g, ZGPU_ERR_ARG, "block_log 16..20");
+  if (nb == 0) return ZGPU_OK;
+  const uint64_t B = 1ull << blog;
+  std::vector<zgpu_block_t> tab(nb);
+  uint64_t co = 0, ro = 0;
+  for (uint32_t b = 0; b < nb; ++b) {
+    if (raw_len[b] > B) return fail(g, ZGPU_ERR_DAMAGED, "block larger than the archive's block size");
+    tab[b].src = co;
+    tab[b].dst = ro;
+    tab[b].raw_len = raw_len[b];
+    tab[b].comp_len = comp_len[b];
+    tab[b].method = method[b];
+    tab[b].pad = 0;
+    co += comp_len[b];
+    ro += raw_len[b];
+  }
+  if (int rc = ensure(g, g->aux, al(co) + al(ro) + al((size_t)nb * 4) + al((size_t)nb * 32))) return rc;
+  uint8_t* d_comp = (uint8_t*)g->aux.p;
+  uint8_t* d_out = d_comp + al(co);
+  uint32_t* d_err = (uint32_t*)(d_out + al(ro));
+  zgpu_block_t* d_tab = (zgpu_block_t*)((uint8_t*)d_err + al((size_t)nb * 4));
+  HIPCHK(g, hipMemcpyAsync(d_comp, comp, co, hipMemcpyHostToDevice, g->stream));
+  HIPCHK(g, hipMemcpyAsync(d_tab, tab.data(), (size_t)nb * 32, hipMemcpyHostToDevice, g->stream));
+  if (int rc = zgpu_decode_blocks(g, d_comp, d_tab, nb, blog, d_out, d_err, nullptr)) return rc;
+  HIPCHK(g, hipMemcpyAsync(err, d_err, (size_t)nb * 4, hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(g, hipMemcpyAsync(out, d_out, ro, hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(g, hipStreamSynchronize(g->stream));
+  for (uint32_t b = 0; b < nb; ++b)
+    if (err[b]) {
+      char m[96];
+      snprintf(m, sizeof m, "block %u damaged (GPU decoder, code 0x%x)", b, err[b]);
+      return fail(g, ZGPU_ERR_DAMAGED, m);
+    }
+  return ZGPU_OK;
+}
+
+int zgpu_crc32(zgpu_t* g, const uint8_t* data, uint64_t n, uint32_t blog, uint32_t* crc) {
+  if (blog < 16 || blog > 20) return fail(g, ZGPU_ERR_ARG, "block_log 16..20");
+  const uint64_t B = 1ull << blog;
+  const uint32_t nb = (uint32_t)((n + B - 1) / B);
+  if (nb == 0) return ZGPU_OK;
+  std::vector<zgpu_block_t> tab(nb);
+  for (uint32_t b = 0; b < nb; ++b) {
+    tab[b] = zgpu_block_t{};
+    tab[b].src = (uint64_t)b * B;
+    tab[b].raw_len = (uint32_t)std::min<uint64_t>(B, n - (uint64_t)b * B);
+  }
+  if (int rc = ensure(g, g->aux, al(n) + al((size_t)nb * 4) + al((size_t)nb * 32))) return rc;
+  uint8_t* d_in = (uint8_t*)g->aux.p;
+  uint32_t* d_crc = (uint32_t*)(d_in + al(n));
+  zgpu_block_t* d_tab = (zgpu_block_t*)((uint8_t*)d_crc + al((size_t)nb * 4));
+  HIPCHK(g, hipMemcpyAsync(d_in, data, n, hipMemcpyHostToDevice, g->stream));
+  HIPCHK(g, hipMemcpyAsync(d_tab, tab.data(), (size_t)nb * 32, hipMemcpyHostToDevice, g->stream));
+  if (int rc = zgpu_crc32_blocks(g, d_in, d_tab, nb, 0, d_crc, nullptr)) return rc;
+  HIPCHK(g, hipMemcpyAsync(crc, d_crc, (size_t)nb * 4, hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(g, hipStreamSynchronize(g->stream));
+  return ZGPU_OK;
+}
+
+}  // extern "C"

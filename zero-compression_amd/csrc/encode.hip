@@ -1,0 +1,739 @@
+// ZB2 block encoder for gfx950. The algorithm is the reference GPU encoder's (kernels/zero.cl:44-618) made
+// deterministic: every bucket is sorted completely by (key, position), so the match candidates of a position are
+// exactly the nearest earlier positions with its 27-bit hash key, whatever order the scatter's atomics produced.
+//
+// Per launch: nb blocks, block b's raw bytes at in + blocks[b].src (blocks[b].raw_len of them). Per-block scratch
+// is indexed by b (see EncScratch in capi.cpp). Stages:
+//   k_enc_hash      hash5 of every position, bucket counts and a scatter into buckets, counters in LDS
+//   k_enc_sort_s    buckets of <= 16 entries sorted in registers; larger ones queued
+//   k_enc_sort_m    queued buckets of <= 4096 entries: bitonic sort in LDS
+//   k_enc_sort_l    larger buckets: a workgroup's stable LSD radix sort through global memory
+//   k_enc_match     the best match of every position among its run check and sorted neighbours
+//   k_enc_lazy      a match gives way to a better one a byte later
+//   k_enc_parse     the greedy chain: segment walkers in parallel, then one lane joins them
+//   k_enc_tokens    the chain -> literals and tokens; groups, recent offsets, tok/ext/sym/groups, offset bits
+//   k_enc_rans      per (block, stream): histogram, 12-bit frequencies, 64 lanes of rANS
+//   k_enc_pack      the payload, or "stored" when it is not smaller than the block
+#include "zcommon.h"
+
+namespace zk {
+
+#define KEY_TAG_BITS 12u
+__host__ __device__ inline uint32_t hbits_of(uint32_t blog) { return blog - 3u < 15u ? blog - 3u : 15u; }
+
+__device__ inline uint32_t ld32(const uint8_t* p) { uint32_t v; __builtin_memcpy(&v, p, 4); return v; }
+__device__ inline uint64_t ld64(const uint8_t* p) { uint64_t v; __builtin_memcpy(&v, p, 8); return v; }
+
+// hash of the 5 bytes at p (kernels/zero.cl:49-58); p + 4 <= n, the byte past the block reads as 0
+__device__ inline uint32_t hash5(const uint8_t* inb, uint32_t n, uint32_t p) {
+  const uint32_t v = ld32(inb + p);
+  const uint32_t w = p + 4 < n ? (uint32_t)inb[p + 4] : 0u;
+  return (v * 2654435761u) ^ (w * 0x9E3779B1u + (w >> 3));
+}
+
+// common prefix of the bytes at a and b (a < b), up to maxlen; b + maxlen <= n keeps every read in the block
+__device__ inline uint32_t prefix_len(const uint8_t* inb, uint32_t a, uint32_t b, uint32_t maxlen) {
+  uint32_t l = 0;
+  while (l + 8 <= maxlen) {
+    const uint64_t x = ld64(inb + a + l) ^ ld64(inb + b + l);
+    if (x) return l + ((uint32_t)__builtin_ctzll(x) >> 3);
+    l += 8;
+  }
+  if (l + 4 <= maxlen) {
+    const uint32_t x = ld32(inb + a + l) ^ ld32(inb + b + l);
+    if (x) return l + ((uint32_t)__builtin_ctz(x) >> 3);
+    l += 4;
+  }
+  while (l < maxlen && inb[a + l] == inb[b + l]) ++l;
+  return l;
+}
+
+__device__ inline uint32_t bitlen(uint32_t d) { return 31u - (uint32_t)__builtin_clz(d); }
+
+// ---------------------------------------------------------------- 1. hash, count, scatter (one workgroup per block)
+// srt[b*B + slot] = (12 tag bits of the key) << blog | position, grouped by bucket (the top hbits of the hash);
+// bstart[b*(HB+1) + h] = the first slot of bucket h.
+__global__ __launch_bounds__(1024) void k_enc_hash(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
+                                                   uint32_t blog, uint32_t* __restrict__ srt,
+                                                   uint32_t* __restrict__ bstart) {
+  extern __shared__ uint32_t cnt[];  // HB counters, then HB/1024 partial sums
+  const uint32_t b = blockIdx.x, tid = threadIdx.x;
+  const uint32_t HBITS = hbits_of(blog), HB = 1u << HBITS;
+  const Block blk = blocks[b];
+  const uint8_t* inb = in + blk.src;
+  const uint32_t n = blk.raw_len;
+  const uint32_t count = n >= 4 ? n - 3 : 0;
+  for (uint32_t i = tid; i < HB; i += 1024) cnt[i] = 0;
+  __syncthreads();
+  for (uint32_t p = tid; p < count; p += 1024) atomicAdd(&cnt[hash5(inb, n, p) >> (32 - HBITS)], 1u);
+  __syncthreads();
+  // exclusive scan of HB counters: each thread sums a run of `per`, one thread scans the partials
+  const uint32_t per = HB / 1024u ? HB / 1024u : 1u;
+  uint32_t* part = cnt + HB;
+  if (tid * per < HB) {
+    uint32_t s = 0;
+    for (uint32_t i = 0; i < per; ++i) s += cnt[tid * per + i];
+    part[tid] = s;
+  }
+  __syncthreads();
+  if (tid < 64) {  // one wavefront scans the (at most 1024) partials, 16 per lane
+    const uint32_t np = HB / per;
+    const uint32_t k = (np + 63) / 64;
+    uint32_t s = 0;
+    for (uint32_t i = 0; i < k; ++i) if (tid * k + i < np) s += part[tid * k + i];
+    uint32_t run = wave_scan_excl(s);
+    for (uint32_t i = 0; i < k; ++i)
+      if (tid * k + i < np) { const uint32_t x = part[tid * k + i]; part[tid * k + i] = run; run += x; }
+  }
+  __syncthreads();
+  uint32_t* bs = bstart + (size_t)b * (HB + 1);
+  if (tid * per < HB) {
+    uint32_t run = part[tid];
+    for (uint32_t i = 0; i < per; ++i) {
+      const uint32_t x = cnt[tid * per + i];
+      cnt[tid * per + i] = run;
+      bs[tid * per + i] = run;
+      run += x;
+    }
+  }
+  if (tid == 0) bs[HB] = count;
+  __syncthreads();
+  uint32_t* sb = srt + ((size_t)b << blog);
+  const uint32_t keybits = HBITS + KEY_TAG_BITS;
+  for (uint32_t p = tid; p < count; p += 1024) {
+    const uint32_t h = hash5(inb, n, p);
+    const uint32_t slot = atomicAdd(&cnt[h >> (32 - HBITS)], 1u);
+    sb[slot] = ((h >> (32 - keybits)) & ((1u << KEY_TAG_BITS) - 1u)) << blog | p;
+  }
+}
+
+// ---------------------------------------------------------------- 2. bucket sorts (the values are distinct)
+struct SortQueues {
+  uint32_t* mid;    // (block, bucket) pairs of 17..4096 entries
+  uint32_t* large;  // pairs of more than 4096 entries
+  uint32_t* counts; // [0] mid, [1] large
+  uint32_t cap;     // pairs each queue can hold
+};
+
+__global__ __launch_bounds__(256) void k_enc_sort_s(const Block* __restrict__ blocks, uint32_t nb, uint32_t blog,
+                                                    uint32_t* __restrict__ srt, const uint32_t* __restrict__ bstart,
+                                                    SortQueues q) {
+  const uint32_t HB = 1u << hbits_of(blog);
+  const uint64_t gid = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  const uint32_t b = (uint32_t)(gid / HB), h = (uint32_t)(gid % HB);
+  if (b >= nb) return;
+  const uint32_t* bs = bstart + (size_t)b * (HB + 1);
+  const uint32_t lo = bs[h], k = bs[h + 1] - lo;
+  if (k < 2) return;
+  if (k > 16) {
+    const uint32_t which = k > 4096 ? 1 : 0;
+    const uint32_t at = atomicAdd(&q.counts[which], 1u);
+    if (at < q.cap) { uint32_t* dst = which ? q.large : q.mid; dst[2 * at] = b; dst[2 * at + 1] = h; }
+    return;
+  }
+  uint32_t* s = srt + ((size_t)b << blog) + lo;
+  uint32_t v[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) v[i] = (uint32_t)i < k ? s[i] : 0xFFFFFFFFu;
+#pragma unroll
+  for (int kk = 2; kk <= 16; kk <<= 1)
+#pragma unroll
+    for (int j = kk >> 1; j > 0; j >>= 1)
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int l = i ^ j;
+        if (l > i) {
+          const bool asc = (i & kk) == 0;
+          const uint32_t a = v[i], c = v[l];
+          if ((a > c) == asc) { v[i] = c; v[l] = a; }
+        }
+      }
+#pragma unroll
+  for (int i = 0; i < 16; ++i)
+    if ((uint32_t)i < k) s[i] = v[i];
+}
+
+// queued buckets of 17..4096 entries: one workgroup per bucket, bitonic sort in LDS (padded to a power of two)
+__global__ __launch_bounds__(256) void k_enc_sort_m(uint32_t blog, uint32_t* __restrict__ srt,
+                                                    const uint32_t* __restrict__ bstart, SortQueues q) {
+  __shared__ uint32_t a[4096];
+  const uint32_t HB = 1u << hbits_of(blog);
+  const uint32_t total = min(q.counts[0], q.cap);
+  for (uint32_t w = blockIdx.x; w < total; w += gridDim.x) {
+    const uint32_t b = q.mid[2 * w], h = q.mid[2 * w + 1];
+    const uint32_t* bs = bstart + (size_t)b * (HB + 1);
+    const uint32_t lo = bs[h], k = bs[h + 1] - lo;
+    uint32_t* s = srt + ((size_t)b << blog) + lo;
+    uint32_t np2 = 32;
+    while (np2 < k) np2 <<= 1;
+    for (uint32_t i = threadIdx.x; i < np2; i += 256) a[i] = i < k ? s[i] : 0xFFFFFFFFu;
+    __syncthreads();
+    for (uint32_t kk = 2; kk <= np2; kk <<= 1)
+      for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
+        for (uint32_t i = threadIdx.x; i < np2; i += 256) {
+          const uint32_t l = i ^ j;
+          if (l > i) {
+            const bool asc = (i & kk) == 0;
+            const uint32_t x = a[i], y = a[l];
+            if ((x > y) == asc) { a[i] = y; a[l] = x; }
+          }
+        }
+        __syncthreads();
+      }
+    for (uint32_t i = threadIdx.x; i < k; i += 256) s[i] = a[i];
+    __syncthreads();
+  }
+}
+
+// queued buckets of more than 4096 entries: one workgroup of 1024 per bucket, a stable LSD radix sort with 8-bit
+// digits through a temporary of the same size (tmp, indexed like srt), in-wavefront ranks from ballots.
+__global__ __launch_bounds__(1024) void k_enc_sort_l(uint32_t blog, uint32_t* __restrict__ srt,
+                                                     uint32_t* __restrict__ tmp, const uint32_t* __restrict__ bstart,
+                                                     SortQueues q) {
+  __shared__ uint32_t base[256], tile_tot[256];
+  __shared__ uint32_t wcnt[16][256];
+  const uint32_t HB = 1u << hbits_of(blog);
+  const uint32_t tid = threadIdx.x, wv = tid >> 6;
+  const uint32_t total = min(q.counts[1], q.cap);
+  for (uint32_t w = blockIdx.x; w < total; w += gridDim.x) {
+    const uint32_t b = q.large[2 * w], h = q.large[2 * w + 1];
+    const uint32_t* bs = bstart + (size_t)b * (HB + 1);
+    const uint32_t lo = bs[h], k = bs[h + 1] - lo;
+    uint32_t* A = srt + ((size_t)b << blog) + lo;
+    uint32_t* T = tmp + ((size_t)b << blog) + lo;
+    for (uint32_t i = tid; i < 16 * 256; i += 1024) (&wcnt[0][0])[i] = 0;
+    for (uint32_t pass = 0; pass < 4; ++pass) {
+      const uint32_t sh = pass * 8;
+      uint32_t* src = (pass & 1) ? T : A;
+      uint32_t* dst = (pass & 1) ? A : T;
+      if (tid < 256) base[tid] = 0;
+      __syncthreads();
+      for (uint32_t i = tid; i < k; i += 1024) atomicAdd(&base[(src[i] >> sh) & 255u], 1u);
+      __syncthreads();
+      if (tid < 64) {  // exclusive scan of 256 counts, 4 per lane
+        uint32_t c4[4], s = 0;
+        for (int j = 0; j < 4; ++j) { c4[j] = base[tid * 4 + j]; s += c4[j]; }
+        uint32_t run = wave_scan_excl(s);
+        for (int j = 0; j < 4; ++j) { base[tid * 4 + j] = run; run += c4[j]; }
+      }
+      __syncthreads();
+      for (uint32_t t0 = 0; t0 < k; t0 += 1024) {
+        const uint32_t i = t0 + tid;
+        const bool valid = i < k;
+        const uint32_t v = valid ? src[i] : 0u;
+        const uint32_t d = (v >> sh) & 255u;
+        uint64_t m = ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+          const uint64_t bb = ballot((d >> bit) & 1u);
+          m &= ((d >> bit) & 1u) ? bb : ~bb;
+        }
+        const uint32_t rank = popc64(m & lanemask_lt());
+        if (valid && rank == 0) wcnt[wv][d] = popc64(m);
+        __syncthreads();
+        if (tid < 256) {
+          uint32_t run = 0;
+          for (int x = 0; x < 16; ++x) { const uint32_t c = wcnt[x][tid]; wcnt[x][tid] = run; run += c; }
+          tile_tot[tid] = run;
+        }
+        __syncthreads();
+        if (valid) dst[base[d] + wcnt[wv][d] + rank] = v;
+        __syncthreads();
+        if (tid < 256) {
+          base[tid] += tile_tot[tid];
+          for (int x = 0; x < 16; ++x) wcnt[x][tid] = 0;
+        }
+        __syncthreads();
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- 3. match finder (one thread per sorted slot)
+__global__ __launch_bounds__(256) void k_enc_match(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
+                                                   uint32_t blog, uint32_t ncand, uint32_t lencap,
+                                                   const uint32_t* __restrict__ srt, const uint32_t* __restrict__ bstart,
+                                                   uint32_t* __restrict__ match) {
+  const uint32_t B = 1u << blog;
+  const uint32_t b = blockIdx.x >> (blog - 8);
+  const uint32_t i = ((blockIdx.x << 8) + threadIdx.x) & (B - 1u);
+  const Block blk = blocks[b];
+  const uint32_t n = blk.raw_len;
+  if (i >= n) return;
+  const uint32_t count = n >= 4 ? n - 3 : 0;
+  uint32_t* mb = match + ((size_t)b << blog);
+  if (i >= count) { mb[i] = 0; return; }  // (the last 3 positions are never in srt: exactly these slots)
+  const uint8_t* inb = in + blk.src;
+  const uint32_t* sb = srt + ((size_t)b << blog);
+  const uint32_t HBITS = hbits_of(blog), HB = 1u << HBITS;
+  const uint32_t e = sb[i];
+  const uint32_t p = e & (B - 1u), tag = e >> blog;
+  if (p >= count) return;  // (never with a correct sort; keeps every read inside the block)
+  const uint32_t bucket = hash5(inb, n, p) >> (32 - HBITS);
+  const uint32_t bfirst = bstart[(size_t)b * (HB + 1) + bucket];
+  const uint32_t pw = ld32(inb + p);
+  const uint32_t maxlen = min(lencap, n - p);
+  uint32_t best = 0, bestd = 0;
+  if (p > 0 && ld32(inb + p - 1) == pw) {  // a run of one byte value (kernels/zero.cl:219-222)
+    best = 4 + prefix_len(inb, p + 3, p + 4, maxlen - 4);
+    bestd = 1;
+  }
+  for (uint32_t k = 1; k <= ncand && best < maxlen; ++k) {
+    if (i < bfirst + k) break;
+    const uint32_t c = sb[i - k];
+    if ((c >> blog) != tag) break;  // (sorted by key: no earlier slot has this key)
+    const uint32_t j = c & (B - 1u);
+    if (j >= p || ld32(inb + j) != pw) continue;
+    if (best >= 4 && inb[j + best] != inb[p + best]) continue;
+    const uint32_t l = 4 + prefix_len(inb, j + 4, p + 4, maxlen - 4);
+    const uint32_t d = p - j;
+    if (l > best || (l == best && d < bestd)) { best = l; bestd = d; }
+  }
+  mb[p] = best >= 4 ? (best << 20) | (bestd - 1) : 0u;
+}
+
+// ---------------------------------------------------------------- 4. lazy matching (kernels/zero.cl:239-255)
+__global__ __launch_bounds__(256) void k_enc_lazy(const Block* __restrict__ blocks, uint32_t blog,
+                                                  const uint32_t* __restrict__ match, uint32_t* __restrict__ lazy) {
+  const uint32_t B = 1u << blog;
+  const uint32_t b = blockIdx.x >> (blog - 8);
+  const uint32_t p = ((blockIdx.x << 8) + threadIdx.x) & (B - 1u);
+  const uint32_t n = blocks[b].raw_len;
+  if (p >= n) return;
+  const size_t at = ((size_t)b << blog) + p;
+  const uint32_t v = match[at];
+  const uint32_t v2 = p + 1 < n ? match[at + 1] : 0u;
+  const int l = (int)(v >> 20), l2 = (int)(v2 >> 20);
+  uint32_t r = v;
+  if (l && l2) {
+    const int c = (int)bitlen((v & 0xFFFFFu) + 1u), c2 = (int)bitlen((v2 & 0xFFFFFu) + 1u);
+    if (8 * l2 - c2 - 8 > 8 * l - c) r = 0;
+  }
+  lazy[at] = r;
+}
+
+// ---------------------------------------------------------------- 5. the greedy parse (one workgroup per block)
+// the length of the match a value holds at p, or 0 when it holds none (a value that could not be the match finder's
+// is taken as none: a bug elsewhere then costs ratio, never an access outside the block)
+__device__ inline uint32_t match_at(uint32_t v, uint32_t p, uint32_t n) {
+  const uint32_t l = v >> 20, d = (v & 0xFFFFFu) + 1u;
+  return l >= 4 && l <= n - p && d <= p ? l : 0u;
+}
+
+// the position after the token at p; a match cut at LENCAP is finished here, up to 4095 bytes (kernels/zero.cl:259-271)
+__device__ inline uint32_t nextpos(uint32_t* m, const uint8_t* inb, uint32_t n, uint32_t lencap, uint32_t p) {
+  const uint32_t v = m[p];
+  uint32_t l = match_at(v, p, n);
+  if (l == lencap) {
+    const uint32_t maxlen = min(4095u, n - p);
+    if (maxlen > l) {
+      const uint32_t j = p - (v & 0xFFFFFu) - 1u;
+      const uint32_t x = l + prefix_len(inb, j + l, p + l, maxlen - l);
+      if (x != l) { m[p] = (x << 20) | (v & 0xFFFFFu); l = x; }
+    }
+  }
+  return p + (l ? l : 1u);
+}
+
+// Each of the 256 threads walks its segment from the segment's start, marking token starts in an LDS bitmap.
+// Thread 0 then follows the true chain: where it meets a walk the two coincide from there on (the chain is a pure
+// function of the position), so only the walk's prefix is wrong; where it never meets one, the walk is wrong whole.
+// mark[b] receives the bitmap of the chain (B bits).
+__global__ __launch_bounds__(256) void k_enc_parse(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
+                                                   uint32_t blog, uint32_t lencap, uint32_t* __restrict__ lazy,
+                                                   uint32_t* __restrict__ mark) {
+  extern __shared__ uint32_t sm[];  // bitmap (B/32 words), then ex[256], then the join's path (SEG entries)
+  const uint32_t B = 1u << blog, SEG = B / 256u;
+  const uint32_t b = blockIdx.x, k = threadIdx.x;
+  uint32_t* bm = sm;
+  uint32_t* ex = sm + B / 32u;
+  uint32_t* path = ex + 256;
+  const Block blk = blocks[b];
+  const uint8_t* inb = in + blk.src;
+  const uint32_t n = blk.raw_len;
+  uint32_t* m = lazy + ((size_t)b << blog);
+  for (uint32_t i = k; i < B / 32u; i += 256) bm[i] = 0;
+  __syncthreads();
+  {
+    const uint32_t s = k * SEG, send = min(s + SEG, n);
+    uint32_t p = s;
+    while (p < send) {
+      bm[p >> 5] |= 1u << (p & 31u);  // (a segment owns whole words: SEG is a multiple of 32)
+      p = nextpos(m, inb, n, lencap, p);
+    }
+    ex[k] = s < n ? p : s;
+  }
+  __syncthreads();
+  if (k == 0) {
+    uint32_t e = 0;
+    for (uint32_t q = 0; q < 256; ++q) {
+      const uint32_t s = q * SEG;
+      if (s >= n) break;
+      const uint32_t send = min(s + SEG, n);
+      uint32_t clear_to = send, np = 0, t = e;
+      if (e < send) {
+        bool merged = false;
+        while (t < send) {
+          if ((bm[t >> 5] >> (t & 31u)) & 1u) { merged = true; break; }
+          path[np++] = t;
+          t = nextpos(m, inb, n, lencap, t);
+        }
+        if (merged) { clear_to = t; e = ex[q]; } else { e = t; }
+      }
+      // the walk's marks before the meeting point are not the chain's
+      for (uint32_t x = s; x < clear_to;) {
+        if ((x & 31u) == 0 && x + 32 <= clear_to) { bm[x >> 5] = 0; x += 32; }
+        else { bm[x >> 5] &= ~(1u << (x & 31u)); ++x; }
+      }
+      for (uint32_t i = 0; i < np; ++i) bm[path[i] >> 5] |= 1u << (path[i] & 31u);
+    }
+  }
+  __syncthreads();
+  uint32_t* mk = mark + ((size_t)b << (blog - 5));
+  for (uint32_t i = k; i < B / 32u; i += 256) mk[i] = bm[i];
+}
+
+// ---------------------------------------------------------------- 6. tokens and streams (one wavefront per block)
+// From the chain: lit (the literal bytes), then per token (in order) its literal run L, length M, distance D, and
+// the groups (src/codec2.h:102-136; kernels/zero.cl:375-451): a group closes at 255 tokens, or before a match whose
+// source reaches into the group (reading only its own literals is allowed). sym is the offset's bit length, or
+// 250..252 for one of the last three offsets as of the end of the group before.
+// cnt[b*8 + 0..4] = tokens, literals, ext bytes, offset-bit bytes, groups.
+struct TokBufs {
+  uint8_t* lit;      // B per block
+  uint32_t* LB;      // literals before each token (TCAP per block)
+  uint32_t* TM;      // match length (TCAP)
+  uint32_t* TD;      // distance (TCAP)
+  uint8_t* tok;      // TCAP
+  uint8_t* sym;      // TCAP
+  uint8_t* grp;      // TCAP
+  uint8_t* ext;      // ECAP
+  uint32_t* offw;    // OCAP/4 + 1 words
+  uint32_t* cnt;     // 8 per block
+};
+
+__global__ __launch_bounds__(64) void k_enc_tokens(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
+                                                   uint32_t blog, const uint32_t* __restrict__ lazy,
+                                                   const uint32_t* __restrict__ mark, TokBufs T) {
+  __shared__ uint32_t stage[48];
+  const Geo G = geo_of(blog);
+  const uint32_t b = blockIdx.x, lane = threadIdx.x;
+  const Block blk = blocks[b];
+  const uint8_t* inb = in + blk.src;
+  const uint32_t n = blk.raw_len;
+  const uint32_t* m = lazy + ((size_t)b << blog);
+  const uint32_t* mk = mark + ((size_t)b << (blog - 5));
+  uint8_t* lit = T.lit + ((size_t)b << blog);
+  uint32_t* LB = T.LB + (size_t)b * G.TCAP;
+  uint32_t* TM = T.TM + (size_t)b * G.TCAP;
+  uint32_t* TD = T.TD + (size_t)b * G.TCAP;
+  // phase A: the chain in 64-position chunks
+  uint32_t nt = 0, nl = 0;
+  for (uint32_t p0 = 0; p0 < n; p0 += 64) {
+    const uint64_t w = (uint64_t)mk[p0 >> 5] | (uint64_t)mk[(p0 >> 5) + 1] << 32;
+    if (!w) continue;
+    const uint32_t p = p0 + lane;
+    const bool on = p < n && ((w >> lane) & 1u);
+    const uint32_t v = on ? m[p] : 0u;
+    const bool istok = on && match_at(v, p, n) != 0;
+    const bool islit = on && !istok;
+    const uint64_t lm = ballot(islit), tm = ballot(istok);
+    const uint32_t lq = nl + popc64(lm & lanemask_lt());
+    const uint32_t tq = nt + popc64(tm & lanemask_lt());
+    if (islit) lit[lq] = inb[p];
+    if (istok) { LB[tq] = lq; TM[tq] = v >> 20; TD[tq] = (v & 0xFFFFFu) + 1u; }
+    nl += popc64(lm);
+    nt += popc64(tm);
+  }
+  wave_fence();
+  // phase B: 64 tokens at a time
+  uint8_t* tokb = T.tok + (size_t)b * G.TCAP;
+  uint8_t* symb = T.sym + (size_t)b * G.TCAP;
+  uint8_t* grpb = T.grp + (size_t)b * G.TCAP;
+  uint8_t* extb = T.ext + (size_t)b * G.ECAP;
+  uint32_t* ow = T.offw + (size_t)b * (G.OCAP / 4u + 1u);
+  uint32_t op = 0, ei = 0, bp = 0, ng = 0, gstart = 0, gt = 0, carry = 0;
+  uint32_t r0 = 1, r1 = 4, r2 = 8, l0 = 1, l1 = 4, l2 = 8;
+  for (uint32_t t = 0; t < nt; t += 64) {
+    const uint32_t c = min(64u, nt - t);
+    const bool on = lane < c;
+    const uint32_t q = t + lane;
+    const uint32_t lb = on ? LB[q] : 0u;
+    const uint32_t lprev = on && q ? LB[q - 1] : 0u;
+    const uint32_t L = lb - lprev;
+    const uint32_t M = on ? TM[q] : 0u;
+    const uint32_t D = on ? TD[q] : 1u;
+    const uint32_t opk = op + wave_scan_excl(L + M);
+    const uint32_t i = opk + L, j = i - D;
+    const uint32_t need = j < opk ? j + min(M, D) : 0u;  // how far the source reaches, unless its own literals
+    uint32_t sv = 0;
+    uint32_t lo = 0;
+    while (lo < c) {
+      const bool viol = on && lane >= lo && ((need > gstart && !(lane == lo && gt == 0)) || gt + (lane - lo) >= 255u);
+      const uint64_t vm = ballot(viol);
+      const uint32_t first = vm ? min((uint32_t)__builtin_ctzll(vm), c) : c;
+      if (lane >= lo && lane < first) sv = D == r0 ? 250u : D == r1 ? 251u : D == r2 ? 252u : bitlen(D);
+      for (uint32_t x = first > lo + 3 ? first - 3 : lo; x < first; ++x) {
+        const uint32_t v = readlane(D, x);
+        l2 = l1; l1 = l0; l0 = v;
+      }
+      gt += first - lo;
+      if (first < c) {
+        if (gt) { if (lane == 0) grpb[ng] = (uint8_t)gt; ++ng; }
+        gt = 0;
+        gstart = readlane(opk, first);
+        r0 = l0; r1 = l1; r2 = l2;
+      }
+      lo = first;
+    }
+    if (on) {
+      symb[q] = (uint8_t)sv;
+      tokb[q] = (uint8_t)((min(L, 15u) << 4) | min(M - 4u, 15u));
+    }
+    const uint32_t fl = on && L >= 15u, fm = on && M >= 19u;
+    const uint32_t xq = ei + wave_scan_excl(3u * fl + 2u * fm);
+    if (fl) { const uint32_t x = L - 15u; extb[xq] = (uint8_t)x; extb[xq + 1] = (uint8_t)(x >> 8); extb[xq + 2] = (uint8_t)(x >> 16); }
+    if (fm) { const uint32_t x = M - 19u, y = xq + 3u * fl; extb[y] = (uint8_t)x; extb[y + 1] = (uint8_t)(x >> 8); }
+    // offset bits, LSB first: gathered in LDS words, then stored whole (the last one carried into the next chunk)
+    const uint32_t nbb = on && sv < 250u ? sv : 0u;
+    const uint32_t bq = bp + wave_scan_excl(nbb);
+    const uint32_t tb = wave_sum(nbb);
+    const uint32_t wb = bp >> 5;
+    if (lane < 48) stage[lane] = 0;
+    __syncthreads();
+    if (nbb) {
+      const uint32_t extra = D - (1u << nbb);
+      const uint32_t w = (bq >> 5) - wb, sh = bq & 31u;
+      atomicOr(&stage[w], extra << sh);
+      if (sh + nbb > 32u) atomicOr(&stage[w + 1], extra >> (32u - sh));
+    }
+    __syncthreads();
+    const uint32_t bend = bp + tb;
+    const uint32_t nw = (bend >> 5) - wb + 1;
+    if (lane < nw) ow[wb + lane] = stage[lane] | (lane == 0 ? carry : 0u);
+    carry = stage[(bend >> 5) - wb] | ((bend >> 5) == wb ? carry : 0u);
+    __syncthreads();
+    op += wave_sum(L + M);
+    ei += wave_sum(3u * fl + 2u * fm);
+    bp = bend;
+  }
+  if (gt) { if (lane == 0) grpb[ng] = (uint8_t)gt; ++ng; }
+  if (lane == 0) {
+    uint32_t* cn = T.cnt + (size_t)b * 8;
+    cn[0] = nt; cn[1] = nl; cn[2] = ei; cn[3] = (bp + 7u) >> 3; cn[4] = ng;
+  }
+}
+
+// ---------------------------------------------------------------- 7. rANS per (block, stream)
+struct RansBufs {
+  uint8_t* lanes;     // lane scratch, LANES bytes per block
+  uint16_t* freq;     // 256 per (block, stream)
+  uint32_t* lanelen;  // 64 per (block, stream)
+  uint32_t* sinfo;    // 4 per (block, stream): kind, section length, present symbols, symbols
+};
+__host__ __device__ inline uint32_t lane_cap(const Geo& G, uint32_t s) {
+  const uint32_t maxn = s == 0 ? G.B : s == 2 ? G.ECAP : G.TCAP;
+  return 2u * ((maxn + 63u) / 64u) + 16u;  // at most 2 bytes per symbol, plus the 4-byte state
+}
+__host__ __device__ inline uint32_t lane_off(const Geo& G, uint32_t s) {
+  uint32_t o = 0;
+  for (uint32_t i = 0; i < s; ++i) o += 64u * lane_cap(G, i);
+  return o;
+}
+__host__ __device__ inline uint32_t lanes_per_block(const Geo& G) { return lane_off(G, 5); }
+
+__device__ inline const uint8_t* stream_syms(const TokBufs& T, const Geo& G, uint32_t b, uint32_t s) {
+  switch (s) {
+    case 0: return T.lit + ((size_t)b << G.blog);
+    case 1: return T.tok + (size_t)b * G.TCAP;
+    case 2: return T.ext + (size_t)b * G.ECAP;
+    case 3: return T.sym + (size_t)b * G.TCAP;
+    default: return T.grp + (size_t)b * G.TCAP;
+  }
+}
+__device__ inline uint32_t stream_count(const uint32_t* cn, uint32_t s) {
+  return s == 0 ? cn[1] : s == 2 ? cn[2] : s == 4 ? cn[4] : cn[0];
+}
+
+__global__ __launch_bounds__(64) void k_enc_rans(uint32_t blog, TokBufs T, RansBufs R) {
+  __shared__ uint32_t hist[256];
+  __shared__ uint32_t fc[256];   // freq | cum << 16
+  __shared__ uint32_t rcp[256];  // ceil(2^(31+s) / freq), s = ceil(log2 freq)
+  const Geo G = geo_of(blog);
+  const uint32_t g = blockIdx.x, b = g / 5u, s = g % 5u, lane = threadIdx.x;
+  const uint32_t* cn = T.cnt + (size_t)b * 8;
+  const uint32_t n = stream_count(cn, s);
+  const uint8_t* sy = stream_syms(T, G, b, s);
+  uint32_t* si = R.sinfo + (size_t)g * 4;
+  if (n == 0) {
+    if (lane == 0) { si[0] = 0; si[1] = 0; si[2] = 0; si[3] = 0; }
+    return;
+  }
+  for (uint32_t i = lane; i < 256; i += 64) hist[i] = 0;
+  __syncthreads();
+  for (uint32_t i = lane; i < n; i += 64) atomicAdd(&hist[sy[i]], 1u);
+  __syncthreads();
+  // 12-bit frequencies exactly as zb::normalize (src/codec.h:41-66), lane owns symbols 4*lane..4*lane+3
+  uint32_t f[4], sum = 0, npres = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t c = hist[lane * 4 + k];
+    f[k] = 0;
+    if (c) { f[k] = (uint32_t)(((uint64_t)c << 12) / n); if (!f[k]) f[k] = 1; ++npres; }
+    sum += f[k];
+  }
+  sum = wave_sum(sum);
+  npres = wave_sum(npres);
+  if (sum > 4096u) {
+    // The reference takes one from the first largest frequency (> 1) per step. After E = sum - 4096 steps every
+    // frequency above T+1 is down to T+1, and the first E' of those at T+1 (by symbol) are down to T, where T is
+    // the largest level with need(T) = sum of max(0, f - T) >= E and E' = E - need(T+1).
+    const uint32_t E = sum - 4096u;
+    uint32_t lo = 1, hi = 4096;  // need(lo) >= E holds at lo = 1 (since present <= 4096)
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) >> 1;
+      uint32_t nd = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) nd += f[k] > mid ? f[k] - mid : 0u;
+      nd = wave_sum(nd);
+      if (nd >= E) lo = mid; else hi = mid - 1;
+    }
+    const uint32_t T1 = lo + 1;
+    uint32_t nd1 = 0, at = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) { nd1 += f[k] > T1 ? f[k] - T1 : 0u; at += f[k] >= T1 ? 1u : 0u; }
+    nd1 = wave_sum(nd1);
+    const uint32_t E2 = E - nd1;
+    uint32_t rank = wave_scan_excl(at);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (f[k] >= T1) { f[k] = rank < E2 ? T1 - 1 : T1; ++rank; }
+  } else if (sum < 4096u) {
+    // the deficit to the first largest
+    uint32_t key = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) key = max(key, f[k] << 8 | (255u - (lane * 4 + k)));
+    key = wave_max(key);
+    const uint32_t best = 255u - (key & 255u);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (lane * 4 + k == best) f[k] += 4096u - sum;
+  }
+  uint32_t fs = f[0] + f[1] + f[2] + f[3];
+  uint32_t c = wave_scan_excl(fs);
+  uint16_t* fo = R.freq + (size_t)g * 256;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t sym = lane * 4 + k;
+    fc[sym] = f[k] | c << 16;
+    fo[sym] = (uint16_t)f[k];
+    if (f[k]) {
+      const uint32_t sh = f[k] > 1 ? 32u - (uint32_t)__builtin_clz(f[k] - 1) : 0u;
+      rcp[sym] = (uint32_t)((((uint64_t)1 << (31 + sh)) + f[k] - 1) / f[k]);
+    }
+    c += f[k];
+  }
+  __syncthreads();
+  // lane j encodes symbols j, j+64, ... from the last one back into its scratch, backwards (src/codec.h:112-126)
+  const uint32_t LC = lane_cap(G, s);
+  uint8_t* end = R.lanes + (size_t)b * lanes_per_block(G) + lane_off(G, s) + (size_t)(lane + 1) * LC;
+  uint8_t* ptr = end;
+  uint32_t x = 1u << 23;
+  if (lane < n) {
+    for (int64_t i = (int64_t)lane + (int64_t)((n - 1 - lane) / 64u) * 64; i >= 0; i -= 64) {
+      const uint32_t e = fc[sy[i]];
+      const uint32_t fq = e & 0xFFFFu;
+      const uint32_t xmax = fq << 19;
+      while (x >= xmax) { *--ptr = (uint8_t)x; x >>= 8; }
+      // exact x / fq for x < 2^31: floor(x * ceil(2^(31+s)/fq) / 2^(31+s)) with 2^s >= fq
+      const uint32_t sh = fq > 1 ? 32u - (uint32_t)__builtin_clz(fq - 1) : 0u;
+      const uint32_t qt = (uint32_t)(((uint64_t)x * rcp[sy[i]]) >> (31 + sh));
+      x = (qt << 12) + (x - qt * fq) + (e >> 16);
+    }
+  }
+  ptr -= 4;
+  ptr[0] = (uint8_t)x; ptr[1] = (uint8_t)(x >> 8); ptr[2] = (uint8_t)(x >> 16); ptr[3] = (uint8_t)(x >> 24);
+  const uint32_t ll = (uint32_t)(end - ptr);
+  R.lanelen[(size_t)g * 64 + lane] = ll;
+  const uint32_t total = 32u + 2u * npres + 256u + wave_sum(ll);
+  if (lane == 0) {
+    const uint32_t kind = total >= n ? 0u : 1u;  // raw when rANS is not smaller (src/codec.h:129-135)
+    si[0] = kind; si[1] = kind ? total : n; si[2] = npres; si[3] = n;
+  }
+}
+
+// ---------------------------------------------------------------- 8. the payload (one workgroup per block)
+// out + b*B receives the ZB2 payload (src/codec2.h:140-151) and outlen[b] its size, or 0xFFFFFFFF when the payload
+// would not be smaller than the block (the block is then stored).
+__device__ inline void wg_copy(uint8_t* d, const uint8_t* s, uint32_t n) {
+  typedef uint32_t v4 __attribute__((ext_vector_type(4)));
+  const uint32_t full = n & ~15u;
+  for (uint32_t i = threadIdx.x * 16; i < full; i += blockDim.x * 16) {
+    v4 v;
+    __builtin_memcpy(&v, s + i, 16);
+    __builtin_memcpy(d + i, &v, 16);
+  }
+  if (full + threadIdx.x < n) d[full + threadIdx.x] = s[full + threadIdx.x];
+}
+
+__global__ __launch_bounds__(256) void k_enc_pack(const Block* __restrict__ blocks, uint32_t blog, TokBufs T,
+                                                  RansBufs R, uint8_t* __restrict__ out, uint32_t* __restrict__ outlen) {
+  __shared__ uint32_t loff[65];
+  const Geo G = geo_of(blog);
+  const uint32_t b = blockIdx.x, tid = threadIdx.x;
+  const uint32_t n = blocks[b].raw_len;
+  const uint32_t* cn = T.cnt + (size_t)b * 8;
+  const uint32_t nob = cn[3];
+  uint32_t off[6];
+  off[0] = 20;
+  for (uint32_t s = 0; s < 5; ++s) off[s + 1] = off[s] + 5 + R.sinfo[((size_t)b * 5 + s) * 4 + 1];
+  const uint32_t total = off[5] + nob;
+  if (total >= n) { if (tid == 0) outlen[b] = 0xFFFFFFFFu; return; }
+  uint8_t* o = out + ((size_t)b << blog);
+  if (tid < 20) {
+    const uint32_t v = tid < 4 ? cn[0] : tid < 8 ? cn[1] : tid < 12 ? cn[2] : tid < 16 ? nob : cn[4];
+    o[tid] = (uint8_t)(v >> (8 * (tid & 3)));
+  }
+  if (tid == 0) outlen[b] = total;
+  for (uint32_t s = 0; s < 5; ++s) {
+    const uint32_t g = b * 5 + s;
+    const uint32_t* si = R.sinfo + (size_t)g * 4;
+    const uint32_t kind = si[0], len = si[1], np = si[2], ns = si[3];
+    uint8_t* d = o + off[s];
+    if (tid < 5) d[tid] = tid == 0 ? (uint8_t)kind : (uint8_t)(len >> (8 * (tid - 1)));
+    const uint8_t* sy = stream_syms(T, G, b, s);
+    if (kind == 0) { wg_copy(d + 5, sy, ns); continue; }
+    const uint16_t* fq = R.freq + (size_t)g * 256;
+    const uint32_t* ll = R.lanelen + (size_t)g * 64;
+    // present bitmap, the frequencies of the present symbols in order, the lane lengths
+    const uint32_t f = fq[tid];
+    const uint64_t pm = ballot(f != 0);
+    if ((tid & 7u) == 0) d[5 + (tid >> 3)] = (uint8_t)(pm >> (tid & 63u) & 0xFFu);
+    __shared__ uint32_t wpres[4];
+    if ((tid & 63u) == 0) wpres[tid >> 6] = popc64(pm);
+    __syncthreads();
+    uint32_t rank = popc64(pm & lanemask_lt());
+    for (uint32_t w = 0; w < (tid >> 6); ++w) rank += wpres[w];
+    if (f) { d[5 + 32 + 2 * rank] = (uint8_t)f; d[5 + 33 + 2 * rank] = (uint8_t)(f >> 8); }
+    if (tid < 64) {
+      const uint32_t l = ll[tid];
+      uint8_t* lp = d + 5 + 32 + 2 * np + 4 * tid;
+      lp[0] = (uint8_t)l; lp[1] = (uint8_t)(l >> 8); lp[2] = (uint8_t)(l >> 16); lp[3] = (uint8_t)(l >> 24);
+      const uint32_t incl = wave_scan_incl(l);
+      loff[tid] = incl - l;
+    }
+    __syncthreads();
+    const uint32_t data0 = 5 + 32 + 2 * np + 256;
+    const uint32_t LC = lane_cap(G, s);
+    const uint8_t* lb = R.lanes + (size_t)b * lanes_per_block(G) + lane_off(G, s);
+    for (uint32_t j = 0; j < 64; ++j) {
+      const uint32_t l = ll[j];
+      wg_copy(d + data0 + loff[j], lb + (size_t)(j + 1) * LC - l, l);
+    }
+    __syncthreads();
+  }
+  const uint8_t* ob = (const uint8_t*)(T.offw + (size_t)b * (G.OCAP / 4u + 1u));
+  wg_copy(o + off[5], ob, nob);
+}
+
+}  // namespace zk

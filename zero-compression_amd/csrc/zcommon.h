@@ -1,0 +1,97 @@
+// Shared device-side definitions of the MI355X (gfx950) block codec: block geometry, the per-block table that
+// every kernel reads, wave64 primitives and byte-level memory helpers.
+//
+// Geometry follows the ZB2 format's caps (kernels/zero.cl:8-27): a block is B = 2^block_log bytes (16..20),
+// at most TCAP = B/4 + 16 tokens (every match is >= 4 bytes), ECAP = B/5 + 64 ext bytes, OCAP = 5B/8 + 8
+// offset-bit bytes.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define ZWAVE 64
+
+namespace zk {
+
+struct Geo {
+  uint32_t blog, B, TCAP, ECAP, OCAP;
+};
+__host__ __device__ inline Geo geo_of(uint32_t blog) {
+  Geo g;
+  g.blog = blog;
+  g.B = 1u << blog;
+  g.TCAP = g.B / 4u + 16u;
+  g.ECAP = g.B / 5u + 64u;
+  g.OCAP = 5u * g.B / 8u + 8u;
+  return g;
+}
+
+// One entry per block of a launch (32 bytes). For decoding: src/comp_len/method locate the block's stored bytes,
+// dst/raw_len where its raw bytes go. For encoding: src/raw_len locate the raw bytes; dst is unused.
+struct Block {
+  uint64_t src;
+  uint64_t dst;
+  uint32_t raw_len;
+  uint32_t comp_len;
+  uint32_t method;
+  uint32_t pad;
+};
+
+// per-block decoder error bits (err[b]); the host reports any as a damaged block
+enum : uint32_t {
+  E_HDR = 1u << 0,     // payload shorter than its 20-byte header
+  E_COUNTS = 1u << 1,  // header counts impossible for the block
+  E_STREAM = 1u << 2,  // stream section cut, wrong kind or wrong raw length
+  E_TABLE = 1u << 3,   // rANS frequency table damaged
+  E_LANE = 1u << 4,    // rANS lane cut short
+  E_TOKEN = 1u << 5,   // a token reads or writes outside its block, or the lengths do not add up
+  E_GROUP = 1u << 6,   // a match reads output of its own group (the format forbids it)
+  E_METHOD = 1u << 7,  // unknown method, or a stored block whose lengths differ
+};
+
+// ---- wave64 primitives
+__device__ inline uint32_t lane_id() { return __lane_id(); }
+__device__ inline uint64_t ballot(bool p) { return __ballot(p); }
+__device__ inline uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
+__device__ inline uint32_t popc64(uint64_t m) { return (uint32_t)__popcll(m); }
+
+// inclusive prefix sum over the 64 lanes
+__device__ inline uint32_t wave_scan_incl(uint32_t v) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(v, o, 64);
+    if (lane >= (uint32_t)o) v += t;
+  }
+  return v;
+}
+__device__ inline uint32_t wave_scan_excl(uint32_t v) { return wave_scan_incl(v) - v; }
+__device__ inline uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ inline uint32_t wave_max(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor(v, o, 64));
+  return v;
+}
+__device__ inline uint32_t wave_min(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor(v, o, 64));
+  return v;
+}
+__device__ inline uint32_t readlane(uint32_t v, uint32_t lane) { return __shfl(v, (int)lane, 64); }
+
+// ordering of global-memory writes and later reads among the lanes of one wavefront (a compiler barrier; the
+// hardware keeps one wavefront's vector memory operations in order)
+__device__ inline void wave_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---- byte-level helpers
+__device__ inline uint32_t ld32u(const uint8_t* p) {  // 4 bytes at any address, little endian
+  return (uint32_t)p[0] | (uint32_t)p[1] << 8 | (uint32_t)p[2] << 16 | (uint32_t)p[3] << 24;
+}
+
+}  // namespace zk

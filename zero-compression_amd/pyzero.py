@@ -1,0 +1,324 @@
+"""Python binding of the MI355X zero block codec (libzero_hip.so, C ABI in include/zero_hip.h) and of the
+synthetic workload generators (libzerogen.so).
+
+The codec object mirrors zero::Codec (reference src/archive.h:56-69): ``compress`` turns whole blocks of a buffer
+into (method, payload) pairs, ``decompress`` turns a list of stored blocks back into their raw bytes. Device-resident
+entry points (``encode_blocks`` / ``decode_blocks`` / ``crc32_blocks``) take device pointers, as the benchmark does.
+
+There is no CPU fallback: without the built library on a GPU machine every call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libzero_hip.so")
+GEN_PATH = os.path.join(HERE, "libzerogen.so")
+
+ZGPU_OK = 0
+ZGPU_ERR_DEVICE = -1
+ZGPU_ERR_ARG = -2
+ZGPU_ERR_DAMAGED = -3
+ZGPU_ERR_NOMEM = -4
+STORED = 0xFFFFFFFF
+
+# every symbol include/zero_hip.h declares (tests check the library exports them all)
+EXPORTS = (
+    "zgpu_open", "zgpu_close", "zgpu_props", "zgpu_last_error", "zgpu_sync",
+    "zgpu_alloc", "zgpu_free", "zgpu_alloc_host", "zgpu_free_host", "zgpu_upload", "zgpu_download", "zgpu_fill",
+    "zgpu_stream_create", "zgpu_stream_destroy", "zgpu_stream_sync",
+    "zgpu_event_create", "zgpu_event_destroy", "zgpu_event_record", "zgpu_event_elapsed_ms",
+    "zgpu_encode_blocks", "zgpu_decode_blocks", "zgpu_crc32_blocks",
+    "zgpu_encode_scratch_bytes", "zgpu_decode_scratch_bytes",
+    "zgpu_compress", "zgpu_decompress", "zgpu_crc32",
+    "zgpu_set_profile", "zgpu_stage_times",
+)
+
+BLOCK_DTYPE = np.dtype([("src", "<u8"), ("dst", "<u8"), ("raw_len", "<u4"), ("comp_len", "<u4"),
+                        ("method", "<u4"), ("pad", "<u4")])
+
+
+class ZeroError(RuntimeError):
+    """A codec failure; ``code`` is the ZGPU_ERR_* value."""
+
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+class DamagedError(ZeroError):
+    """At least one block is damaged (the reference's zb::Error, src/codec.h:29-31)."""
+
+
+class DeviceError(ZeroError):
+    """The GPU failed (the reference's gpu::Error, src/gpu.h:17-27)."""
+
+
+class Props(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 128), ("arch", ctypes.c_char * 64), ("cu_count", ctypes.c_int),
+                ("clock_khz", ctypes.c_int), ("hbm_bytes", ctypes.c_uint64), ("lds_per_block", ctypes.c_uint32),
+                ("max_group", ctypes.c_uint32), ("integrated", ctypes.c_int)]
+
+
+_lib = None
+_gen = None
+vp = ctypes.c_void_p
+u32 = ctypes.c_uint32
+u64 = ctypes.c_uint64
+i32 = ctypes.c_int
+
+
+def load_library(path=LIB_PATH):
+    """Loads libzero_hip.so; raises when it is not built (there is no other code path)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ZeroError(ZGPU_ERR_DEVICE, f"{path} is not built: run __graft_entry__.build()")
+    L = ctypes.CDLL(path)
+    sig = {
+        "zgpu_open": (i32, [i32, ctypes.POINTER(vp)]), "zgpu_close": (None, [vp]),
+        "zgpu_props": (i32, [vp, ctypes.POINTER(Props)]), "zgpu_last_error": (ctypes.c_char_p, [vp]),
+        "zgpu_sync": (i32, [vp]),
+        "zgpu_alloc": (i32, [vp, ctypes.c_size_t, ctypes.POINTER(vp)]), "zgpu_free": (i32, [vp, vp]),
+        "zgpu_alloc_host": (i32, [vp, ctypes.c_size_t, ctypes.POINTER(vp)]), "zgpu_free_host": (i32, [vp, vp]),
+        "zgpu_upload": (i32, [vp, vp, vp, ctypes.c_size_t]), "zgpu_download": (i32, [vp, vp, vp, ctypes.c_size_t]),
+        "zgpu_fill": (i32, [vp, vp, ctypes.c_uint8, ctypes.c_size_t, vp]),
+        "zgpu_stream_create": (i32, [vp, ctypes.POINTER(vp)]), "zgpu_stream_destroy": (i32, [vp, vp]),
+        "zgpu_stream_sync": (i32, [vp, vp]),
+        "zgpu_event_create": (i32, [vp, ctypes.POINTER(vp)]), "zgpu_event_destroy": (i32, [vp, vp]),
+        "zgpu_event_record": (i32, [vp, vp, vp]),
+        "zgpu_event_elapsed_ms": (i32, [vp, vp, vp, ctypes.POINTER(ctypes.c_float)]),
+        "zgpu_encode_blocks": (i32, [vp, vp, vp, u32, u32, i32, vp, vp, vp]),
+        "zgpu_decode_blocks": (i32, [vp, vp, vp, u32, u32, vp, vp, vp]),
+        "zgpu_crc32_blocks": (i32, [vp, vp, vp, u32, i32, vp, vp]),
+        "zgpu_encode_scratch_bytes": (ctypes.c_size_t, [u32, u32]),
+        "zgpu_decode_scratch_bytes": (ctypes.c_size_t, [u32, u32]),
+        "zgpu_compress": (i32, [vp, vp, u64, u32, i32, vp, vp]),
+        "zgpu_decompress": (i32, [vp, vp, vp, vp, vp, u32, u32, vp, vp]),
+        "zgpu_crc32": (i32, [vp, vp, u64, u32, vp]),
+        "zgpu_set_profile": (i32, [vp, i32]),
+        "zgpu_stage_times": (i32, [vp, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_float), i32]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    """Address of a numpy array's data (contiguous)."""
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class DeviceBuffer:
+    def __init__(self, gpu, nbytes):
+        self.gpu = gpu
+        self.nbytes = int(nbytes)
+        p = vp()
+        gpu._check(gpu.lib.zgpu_alloc(gpu.h, max(1, self.nbytes), ctypes.byref(p)))
+        self.ptr = p
+
+    def upload(self, arr, offset=0):
+        a = np.ascontiguousarray(arr).view(np.uint8).reshape(-1)
+        assert offset + a.size <= self.nbytes
+        self.gpu._check(self.gpu.lib.zgpu_upload(self.gpu.h, vp(self.ptr.value + offset), _ptr(a), a.size))
+
+    def download(self, nbytes=None, offset=0, dtype=np.uint8):
+        n = self.nbytes - offset if nbytes is None else int(nbytes)
+        out = np.empty(n, dtype=np.uint8)
+        self.gpu._check(self.gpu.lib.zgpu_download(self.gpu.h, _ptr(out), vp(self.ptr.value + offset), n))
+        return out.view(dtype)
+
+    def addr(self, offset=0):
+        return vp(self.ptr.value + offset)
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            self.gpu.lib.zgpu_free(self.gpu.h, self.ptr)
+        self.ptr = None
+
+
+class Gpu:
+    """A device context (one host thread at a time)."""
+
+    def __init__(self, ordinal=0):
+        self.lib = load_library()
+        h = vp()
+        rc = self.lib.zgpu_open(ordinal, ctypes.byref(h))
+        if rc != ZGPU_OK:
+            raise DeviceError(rc, "cannot open the GPU")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.zgpu_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc == ZGPU_OK:
+            return
+        msg = self.lib.zgpu_last_error(self.h).decode(errors="replace")
+        if rc == ZGPU_ERR_DAMAGED:
+            raise DamagedError(rc, msg)
+        if rc == ZGPU_ERR_DEVICE:
+            raise DeviceError(rc, msg)
+        raise ZeroError(rc, msg)
+
+    def props(self):
+        p = Props()
+        self._check(self.lib.zgpu_props(self.h, ctypes.byref(p)))
+        return {"name": p.name.decode(), "arch": p.arch.decode(), "cu_count": p.cu_count,
+                "clock_khz": p.clock_khz, "hbm_bytes": p.hbm_bytes, "lds_per_block": p.lds_per_block}
+
+    # ---- host-buffer forms (zero::Codec)
+    def compress(self, data, block_log=18, level=5):
+        """[(method, payload bytes)] per block: method 1 with the ZB2 payload, or 0 (stored, payload empty)."""
+        a = np.frombuffer(bytes(data), dtype=np.uint8) if not isinstance(data, np.ndarray) else data
+        n = a.size
+        B = 1 << block_log
+        nb = (n + B - 1) // B
+        if nb == 0:
+            return []
+        out = np.empty(nb * B, dtype=np.uint8)
+        lens = np.empty(nb, dtype=np.uint32)
+        self._check(self.lib.zgpu_compress(self.h, _ptr(a), n, block_log, level, _ptr(out), _ptr(lens)))
+        res = []
+        for b in range(nb):
+            if lens[b] == STORED:
+                res.append((0, b""))
+            else:
+                res.append((1, out[b * B: b * B + int(lens[b])].tobytes()))
+        return res
+
+    def decompress(self, blocks, block_log):
+        """blocks: [(method, raw_len, comp bytes)] -> the raw bytes back to back. Raises DamagedError."""
+        nb = len(blocks)
+        if nb == 0:
+            return b""
+        comp = np.frombuffer(b"".join(bytes(c) for _, _, c in blocks) or b"\0", dtype=np.uint8)
+        clen = np.array([len(c) for _, _, c in blocks], dtype=np.uint32)
+        meth = np.array([m for m, _, _ in blocks], dtype=np.uint8)
+        rlen = np.array([r for _, r, _ in blocks], dtype=np.uint32)
+        out = np.empty(max(1, int(rlen.sum(dtype=np.uint64))), dtype=np.uint8)
+        err = np.zeros(nb, dtype=np.uint32)
+        rc = self.lib.zgpu_decompress(self.h, _ptr(comp), _ptr(clen), _ptr(meth), _ptr(rlen), nb, block_log,
+                                      _ptr(out), _ptr(err))
+        self.last_err = err
+        self._check(rc)
+        return out[: int(rlen.sum(dtype=np.uint64))].tobytes()
+
+    def crc32_blocks_host(self, data, block_log):
+        a = np.frombuffer(bytes(data), dtype=np.uint8) if not isinstance(data, np.ndarray) else data
+        B = 1 << block_log
+        nb = (a.size + B - 1) // B
+        crc = np.zeros(max(1, nb), dtype=np.uint32)
+        if nb:
+            self._check(self.lib.zgpu_crc32(self.h, _ptr(a), a.size, block_log, _ptr(crc)))
+        return crc[:nb]
+
+    # ---- device-resident forms
+    def alloc(self, nbytes):
+        return DeviceBuffer(self, nbytes)
+
+    def encode_blocks(self, d_in, d_blocks, nb, block_log, level, d_out, d_out_len, stream=None):
+        self._check(self.lib.zgpu_encode_blocks(self.h, d_in, d_blocks, nb, block_log, level, d_out, d_out_len,
+                                                stream))
+
+    def decode_blocks(self, d_comp, d_blocks, nb, block_log, d_out, d_err, stream=None):
+        self._check(self.lib.zgpu_decode_blocks(self.h, d_comp, d_blocks, nb, block_log, d_out, d_err, stream))
+
+    def crc32_blocks(self, d_data, d_blocks, nb, use_dst, d_crc, stream=None):
+        self._check(self.lib.zgpu_crc32_blocks(self.h, d_data, d_blocks, nb, int(use_dst), d_crc, stream))
+
+    def sync(self):
+        self._check(self.lib.zgpu_sync(self.h))
+
+    def event(self):
+        e = vp()
+        self._check(self.lib.zgpu_event_create(self.h, ctypes.byref(e)))
+        return e
+
+    def record(self, ev, stream=None):
+        self._check(self.lib.zgpu_event_record(self.h, ev, stream))
+
+    def elapsed_ms(self, a, b):
+        ms = ctypes.c_float()
+        self._check(self.lib.zgpu_event_elapsed_ms(self.h, a, b, ctypes.byref(ms)))
+        return ms.value
+
+    def set_profile(self, on):
+        self._check(self.lib.zgpu_set_profile(self.h, int(on)))
+
+    def stage_times(self):
+        names = (ctypes.c_char_p * 32)()
+        ms = (ctypes.c_float * 32)()
+        k = self.lib.zgpu_stage_times(self.h, names, ms, 32)
+        if k < 0:
+            self._check(k)
+        return [(names[i].decode(), ms[i]) for i in range(k)]
+
+
+def block_table(raw_lens, comp_lens=None, methods=None, src=None, dst=None):
+    """A zgpu_block_t array: src/dst default to back-to-back placement of comp (or raw) and raw bytes."""
+    raw = np.asarray(raw_lens, dtype=np.uint64)
+    t = np.zeros(raw.size, dtype=BLOCK_DTYPE)
+    t["raw_len"] = raw
+    comp = raw if comp_lens is None else np.asarray(comp_lens, dtype=np.uint64)
+    t["comp_len"] = comp if comp_lens is not None else 0
+    t["method"] = 0 if methods is None else np.asarray(methods)
+    t["src"] = np.concatenate([[0], np.cumsum(comp)[:-1]]) if src is None else src
+    t["dst"] = np.concatenate([[0], np.cumsum(raw)[:-1]]) if dst is None else dst
+    return t
+
+
+# ---- synthetic workloads (csrc/gen.cpp)
+def _gen_lib():
+    global _gen
+    if _gen is None:
+        if not os.path.exists(GEN_PATH):
+            raise ZeroError(ZGPU_ERR_ARG, f"{GEN_PATH} is not built")
+        g = ctypes.CDLL(GEN_PATH)
+        for name in ("zgen_lz", "zgen_random", "zgen_exec"):
+            getattr(g, name).argtypes = [u64, vp, u64]
+            getattr(g, name).restype = None
+        g.zgen_lognormal_sizes.argtypes = [u64, ctypes.c_double, ctypes.c_double, u64, u64, vp, u64]
+        g.zgen_lognormal_sizes.restype = None
+        _gen = g
+    return _gen
+
+
+def gen_lz(seed, n):
+    """LZ-friendly bytes: literal runs (geometric(16), Zipf(1.1) bytes) and back-copies (4+geometric(20) <= 258 at
+    1+geometric(4096) <= 2^18), each with p = 0.5."""
+    out = np.empty(n, dtype=np.uint8)
+    _gen_lib().zgen_lz(seed, _ptr(out), n)
+    return out
+
+
+def gen_random(seed, n):
+    out = np.empty(n, dtype=np.uint8)
+    _gen_lib().zgen_random(seed, _ptr(out), n)
+    return out
+
+
+def gen_exec(seed, n):
+    """4 KiB pages: 30 % zeros, 40 % uint32 arrays with N(0, 64) deltas, 30 % copies of earlier spans."""
+    out = np.empty(n, dtype=np.uint8)
+    _gen_lib().zgen_exec(seed, _ptr(out), n)
+    return out
+
+
+def gen_lognormal_sizes(seed, median, sigma, lo, hi, count):
+    out = np.empty(count, dtype=np.uint64)
+    _gen_lib().zgen_lognormal_sizes(seed, median, sigma, lo, hi, _ptr(out), count)
+    return out
