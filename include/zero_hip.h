@@ -107,13 +107,16 @@ size_t zgpu_decode_scratch_bytes(uint32_t nb, uint32_t block_log);
 /* ---- the codec on host buffers: zero::Codec::compress / decompress (src/archive.h:56-69) as GpuCodec does them
  * (upload, kernels, download; synchronous). */
 /* n bytes = whole blocks, the last may be shorter. out: (nb << block_log) bytes, block b's payload at b << block_log;
- * out_len[b] = payload size, or ZGPU_STORED. */
+ * out_len[b] = payload size, or ZGPU_STORED. crc (may be NULL): the CRC-32 of each block's raw bytes, which the
+ * .zero container stores per block (src/archive.h:376). */
 int zgpu_compress(zgpu_t* g, const uint8_t* in, uint64_t n, uint32_t block_log, int level, uint8_t* out,
-                  uint32_t* out_len);
+                  uint32_t* out_len, uint32_t* crc);
 /* blocks back to back in comp (comp_len[b] bytes each), their raw bytes back to back into out. err[b] as above;
- * returns ZGPU_ERR_DAMAGED when any block is damaged. */
+ * returns ZGPU_ERR_DAMAGED when any block is damaged. crc (may be NULL): the CRC-32 of each decoded block, for the
+ * container's check (src/archive.h:618). */
 int zgpu_decompress(zgpu_t* g, const uint8_t* comp, const uint32_t* comp_len, const uint8_t* method,
-                    const uint32_t* raw_len, uint32_t nb, uint32_t block_log, uint8_t* out, uint32_t* err);
+                    const uint32_t* raw_len, uint32_t nb, uint32_t block_log, uint8_t* out, uint32_t* err,
+                    uint32_t* crc);
 /* crc[b] of the nb blocks of 2^block_log in data[0..n) (host buffer) */
 int zgpu_crc32(zgpu_t* g, const uint8_t* data, uint64_t n, uint32_t block_log, uint32_t* crc);
 

@@ -146,7 +146,7 @@ DecLayout dec_layout(uint32_t nb, uint32_t blog) {
   const Geo G = geo_of(blog);
   DecLayout L;
   size_t o = 0;
-  L.lit = o; o += al((size_t)nb << blog);
+  L.lit = o; o += al(((size_t)nb << blog) + 64);  // (slack: the replay reads literals 16 bytes at a time)
   L.tok = o; o += al((size_t)nb * G.TCAP);
   L.ext = o; o += al((size_t)nb * G.ECAP);
   L.sym = o; o += al((size_t)nb * G.TCAP);
@@ -383,7 +383,7 @@ int zgpu_crc32_blocks(zgpu_t* g, const uint8_t* d_data, const zgpu_block_t* d_bl
 
 // ---------------------------------------------------------------- host-buffer forms
 int zgpu_compress(zgpu_t* g, const uint8_t* in, uint64_t n, uint32_t blog, int level, uint8_t* out,
-                  uint32_t* out_len) {
+                  uint32_t* out_len, uint32_t* crc) {
   if (blog < 16 || blog > 20 || level < 1 || level > 9) return fail(g, ZGPU_ERR_ARG, "block_log 16..20, level 1..9");
   const uint64_t B = 1ull << blog;
   const uint32_t nb = (uint32_t)((n + B - 1) / B);
@@ -398,14 +398,19 @@ int zgpu_compress(zgpu_t* g, const uint8_t* in, uint64_t n, uint32_t blog, int l
     tab[b].pad = 0;
   }
   const size_t outb = (size_t)nb * B;
-  if (int rc = ensure(g, g->aux, al(n) + al(outb) + al((size_t)nb * 4) + al((size_t)nb * 32))) return rc;
+  if (int rc = ensure(g, g->aux, al(n) + al(outb) + 2 * al((size_t)nb * 4) + al((size_t)nb * 32))) return rc;
   uint8_t* d_in = (uint8_t*)g->aux.p;
   uint8_t* d_out = d_in + al(n);
   uint32_t* d_len = (uint32_t*)(d_out + al(outb));
-  zgpu_block_t* d_tab = (zgpu_block_t*)((uint8_t*)d_len + al((size_t)nb * 4));
+  uint32_t* d_crc = (uint32_t*)((uint8_t*)d_len + al((size_t)nb * 4));
+  zgpu_block_t* d_tab = (zgpu_block_t*)((uint8_t*)d_crc + al((size_t)nb * 4));
   HIPCHK(g, hipMemcpyAsync(d_in, in, n, hipMemcpyHostToDevice, g->stream));
   HIPCHK(g, hipMemcpyAsync(d_tab, tab.data(), (size_t)nb * 32, hipMemcpyHostToDevice, g->stream));
   if (int rc = zgpu_encode_blocks(g, d_in, d_tab, nb, blog, level, d_out, d_len, nullptr)) return rc;
+  if (crc) {
+    if (int rc = zgpu_crc32_blocks(g, d_in, d_tab, nb, 0, d_crc, nullptr)) return rc;
+    HIPCHK(g, hipMemcpyAsync(crc, d_crc, (size_t)nb * 4, hipMemcpyDeviceToHost, g->stream));
+  }
   HIPCHK(g, hipMemcpyAsync(out_len, d_len, (size_t)nb * 4, hipMemcpyDeviceToHost, g->stream));
   HIPCHK(g, hipStreamSynchronize(g->stream));
   // the payloads: one read of the slabs from the first to the last compressed block (src/gpu_codec.h:210-218)
@@ -424,7 +429,7 @@ int zgpu_compress(zgpu_t* g, const uint8_t* in, uint64_t n, uint32_t blog, int l
 }
 
 int zgpu_decompress(zgpu_t* g, const uint8_t* comp, const uint32_t* comp_len, const uint8_t* method,
-                    const uint32_t* raw_len, uint32_t nb, uint32_t blog, uint8_t* out, uint32_t* err) {
+                    const uint32_t* raw_len, uint32_t nb, uint32_t blog, uint8_t* out, uint32_t* err, uint32_t* crc) {
   if (blog < 16 || blog > 20) return fail(g, ZGPU_ERR_ARG, "block_log 16..20");
   if (nb == 0) return ZGPU_OK;
   const uint64_t B = 1ull << blog;
@@ -441,14 +446,19 @@ int zgpu_decompress(zgpu_t* g, const uint8_t* comp, const uint32_t* comp_len, co
     co += comp_len[b];
     ro += raw_len[b];
   }
-  if (int rc = ensure(g, g->aux, al(co) + al(ro) + al((size_t)nb * 4) + al((size_t)nb * 32))) return rc;
+  if (int rc = ensure(g, g->aux, al(co) + al(ro) + 2 * al((size_t)nb * 4) + al((size_t)nb * 32))) return rc;
   uint8_t* d_comp = (uint8_t*)g->aux.p;
   uint8_t* d_out = d_comp + al(co);
   uint32_t* d_err = (uint32_t*)(d_out + al(ro));
-  zgpu_block_t* d_tab = (zgpu_block_t*)((uint8_t*)d_err + al((size_t)nb * 4));
+  uint32_t* d_crc = (uint32_t*)((uint8_t*)d_err + al((size_t)nb * 4));
+  zgpu_block_t* d_tab = (zgpu_block_t*)((uint8_t*)d_crc + al((size_t)nb * 4));
   HIPCHK(g, hipMemcpyAsync(d_comp, comp, co, hipMemcpyHostToDevice, g->stream));
   HIPCHK(g, hipMemcpyAsync(d_tab, tab.data(), (size_t)nb * 32, hipMemcpyHostToDevice, g->stream));
   if (int rc = zgpu_decode_blocks(g, d_comp, d_tab, nb, blog, d_out, d_err, nullptr)) return rc;
+  if (crc) {
+    if (int rc = zgpu_crc32_blocks(g, d_out, d_tab, nb, 1, d_crc, nullptr)) return rc;
+    HIPCHK(g, hipMemcpyAsync(crc, d_crc, (size_t)nb * 4, hipMemcpyDeviceToHost, g->stream));
+  }
   HIPCHK(g, hipMemcpyAsync(err, d_err, (size_t)nb * 4, hipMemcpyDeviceToHost, g->stream));
   HIPCHK(g, hipMemcpyAsync(out, d_out, ro, hipMemcpyDeviceToHost, g->stream));
   HIPCHK(g, hipStreamSynchronize(g->stream));

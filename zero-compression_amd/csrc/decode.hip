@@ -11,18 +11,12 @@
 
 namespace zk {
 
-// exactly n bytes from s to d (16 bytes at a time, then the tail); the ranges must not overlap
-__device__ inline void copy_exact(uint8_t* d, const uint8_t* s, uint32_t n) {
-  typedef uint32_t v4 __attribute__((ext_vector_type(4)));
-  uint32_t i = 0;
-  for (; i + 16 <= n; i += 16) {
-    v4 v;
-    __builtin_memcpy(&v, s + i, 16);
-    __builtin_memcpy(d + i, &v, 16);
-  }
-  if (n - i >= 8) { uint64_t v; __builtin_memcpy(&v, s + i, 8); __builtin_memcpy(d + i, &v, 8); i += 8; }
-  if (n - i >= 4) { uint32_t v; __builtin_memcpy(&v, s + i, 4); __builtin_memcpy(d + i, &v, 4); i += 4; }
-  for (; i < n; ++i) d[i] = s[i];
+// exactly n (0..8) bytes at s, little endian, nothing past s + n read
+__device__ inline uint64_t load8n(const uint8_t* s, uint32_t n) {
+  if (n >= 8) { uint64_t v; __builtin_memcpy(&v, s, 8); return v; }
+  uint64_t v = 0;
+  for (uint32_t k = 0; k < n; ++k) v |= (uint64_t)s[k] << (8 * k);
+  return v;
 }
 
 // a wavefront copies n bytes together (16 bytes per lane per step)
@@ -36,28 +30,6 @@ __device__ inline void wave_copy(uint8_t* d, const uint8_t* s, uint32_t n) {
     __builtin_memcpy(d + i, &v, 16);
   }
   if (full + lane < n) d[full + lane] = s[full + lane];
-}
-
-// the M bytes of a match whose first D bytes are at sp (D < M: the pattern repeats). out_m is where the match goes.
-// Bytes of this match already written are read back from out_m by the same lane.
-__device__ inline void copy_match(uint8_t* out_m, const uint8_t* sp, uint32_t D, uint32_t M) {
-  if (D >= M) { copy_exact(out_m, sp, M); return; }
-  if (D >= 16) {
-    copy_exact(out_m, sp, D);
-    for (uint32_t i = D; i < M; i += 16) {
-      const uint32_t c = M - i < 16 ? M - i : 16;
-      copy_exact(out_m + i, out_m + i - D, c);  // (the source ends at or before i: already written)
-    }
-    return;
-  }
-  // D < 16: a 16-byte register holding the pattern from its start, written every S bytes (S a multiple of D)
-  uint8_t pat[16];
-#pragma unroll
-  for (uint32_t k = 0; k < 16; ++k) pat[k] = sp[k % D];
-  const uint32_t S = (16 / D) * D;
-  uint32_t i = 0;
-  for (; i + 16 <= M; i += S) __builtin_memcpy(out_m + i, pat, 16);
-  for (uint32_t k = 0; i + k < M; ++k) out_m[i + k] = pat[k];
 }
 
 struct Sections {
@@ -175,33 +147,213 @@ __global__ __launch_bounds__(64) void k_dec_streams(const Block* __restrict__ bl
     }
   }
   __syncthreads();
-  // 64 lanes decode side by side: lane j holds symbols j, j+64, ...
-  const uint8_t* in = body + lstart[lane];
-  const uint8_t* in_end = body + lstart[lane + 1];
-  uint32_t x = ld32u(in);
-  in += 4;
-  bool bad = false;
+  // 64 lanes decode side by side: lane j holds symbols j, j+64, ... A step needs 0, 1 or 2 more bytes of the lane
+  // (x >= 2^11 after the step: 2^15 <= x < 2^23 takes one, x < 2^15 two), read with one 2-byte load from the
+  // lane's position (cached), so the loop has no branch; a lane that reads past its end is damaged.
+  const uint8_t* lp = body + lstart[lane];
+  const uint32_t lnl = lstart[lane + 1] - lstart[lane];
+  const uint8_t* lim = p + blk.comp_len;
+  uint32_t x = ld32u(lp);
+  uint32_t pos = 4;
   for (uint32_t i = lane; i < n; i += 64) {
     const uint32_t sl = x & 4095u;
     const uint32_t sy = slot[sl];
     const uint32_t e = fc[sy];
     dest[i] = (uint8_t)sy;
     x = (e & 0xFFFFu) * (x >> 12) + sl - (e >> 16);
-    while (x < (1u << 23)) {
-      if (in >= in_end) { bad = true; break; }
-      x = (x << 8) | *in++;
+    const uint32_t need = (x < (1u << 15) ? 1u : 0u) + (x < (1u << 23) ? 1u : 0u);
+    const uint8_t* q = lp + pos;
+    uint32_t b01;
+    if (q + 2 <= lim) {
+      uint16_t h;
+      __builtin_memcpy(&h, q, 2);
+      b01 = h;
+    } else {
+      b01 = (q < lim ? (uint32_t)q[0] : 0u) | (q + 1 < lim ? (uint32_t)q[1] << 8 : 0u);
     }
-    if (bad) break;
+    const uint32_t b0 = b01 & 0xFFu, b1 = b01 >> 8;
+    x = need == 0 ? x : need == 1 ? (x << 8) | b0 : (x << 16) | (b0 << 8) | b1;
+    pos += need;
   }
-  if (ballot(bad) && lane == 0) atomicOr(&err[b], E_LANE);
+  if (ballot(pos > lnl) && lane == 0) atomicOr(&err[b], E_LANE);
 }
 
 // ---------------------------------------------------------------- the LZ replay: one wavefront per block
+// 16 bytes as two little-endian halves
+struct U128 {
+  uint64_t lo, hi;
+};
+__device__ inline U128 ld128(const uint8_t* p) { U128 r; __builtin_memcpy(&r, p, 16); return r; }
+__device__ inline void st128(uint8_t* p, const U128& v) { __builtin_memcpy(p, &v, 16); }
+// bytes [0, off) of a, then the first 16 - off bytes of b (off 0..16)
+__device__ inline U128 splice(const U128& a, const U128& b, uint32_t off) {
+  const uint32_t s = off * 8u;
+  const uint32_t s63 = s & 63u, r63 = (64u - s) & 63u;
+  const uint64_t bl = s == 0 ? b.lo : s < 64 ? b.lo << s63 : 0ull;
+  const uint64_t bh = s == 0 ? b.hi : s < 64 ? (b.hi << s63) | (b.lo >> r63) : s < 128 ? b.lo << s63 : 0ull;
+  const uint64_t ml = s == 0 ? 0ull : s < 64 ? ~0ull >> r63 : ~0ull;
+  const uint64_t mh = s <= 64 ? 0ull : s < 128 ? ~0ull >> ((128u - s) & 63u) : ~0ull;
+  return U128{(a.lo & ml) | (bl & ~ml), (a.hi & mh) | (bh & ~mh)};
+}
+// a % d for a < 2^21, d >= 1, by the hardware reciprocal (1 ulp: the quotient is off by at most one, then corrected)
+__device__ inline uint32_t mod_small(uint32_t a, uint32_t d) {
+  const uint32_t q = (uint32_t)((float)a * __builtin_amdgcn_rcpf((float)d));
+  int32_t r = (int32_t)(a - q * d);
+  r += r < 0 ? (int32_t)d : 0;
+  r -= r >= (int32_t)d ? (int32_t)d : 0;
+  return (uint32_t)r;
+}
+__device__ inline U128 shr128(const U128& v, uint32_t nbytes) {  // toward byte 0, nbytes 0..15
+  const uint32_t s = nbytes * 8u, s63 = s & 63u, r63 = (64u - s) & 63u;
+  if (s == 0) return v;
+  if (s < 64) return U128{(v.lo >> s63) | (v.hi << r63), v.hi >> s63};
+  return U128{v.hi >> s63, 0ull};
+}
+// 16 bytes of a pattern of period D < 16 whose first D bytes are in p, starting at period byte ph
+__device__ inline U128 pattern16(U128 e, uint32_t ph, uint32_t D) {
+  for (uint32_t len = D; len < 16; len *= 2) e = splice(e, e, len);  // e[j] = period[j % D]
+  // out[j] = e[ph + j] for j < 16 - ph, then e[ph + j - D] (one period back), which is byte j - (16 - ph) of e
+  // shifted down by 16 - D
+  return splice(shr128(e, ph), shr128(e, 16u - D), 16u - ph);
+}
+
+// where the bytes from output offset x on (token k) come from: a run of `len` bytes read at `src` (kind 0), a short
+// pattern (kind 1: src is its period, `phase` the period byte at x), or the exact path (kind 2)
+struct Run {
+  const uint8_t* src;
+  uint32_t len, kind, phase, period;
+};
+__device__ inline Run run_at(uint8_t* o, const uint8_t* litb, uint32_t raw, uint32_t opk, uint32_t L, uint32_t M,
+                             uint32_t D, uint32_t lik, uint32_t x, uint32_t pend) {
+  const uint32_t ms = opk + L, me = ms + M;
+  Run r;
+  r.kind = 0;
+  r.phase = 0;
+  r.period = D;
+  if (x < ms) {
+    r.len = min(pend, ms) - x;
+    r.src = litb + lik + (x - opk);
+    return r;
+  }
+  uint32_t len = min(pend, me) - x;
+  uint32_t xs = x - D;  // (D >= M: the source of byte x)
+  if (D < M) {
+    const uint32_t ph = mod_small(x - ms, D);
+    if (D < 16) {
+      const uint32_t src = ms - D;
+      r.kind = src >= opk || ms <= opk ? 1u : 2u;
+      r.phase = ph;
+      r.len = len;
+      r.src = src >= opk ? litb + lik + (src - opk) : o + src;
+      if (r.kind == 1 && src < opk && src + 16 > raw) r.kind = 2;
+      return r;
+    }
+    xs = ms - D + ph;
+    len = min(len, D - ph);
+  }
+  if (xs >= opk) {
+    r.src = litb + lik + (xs - opk);
+  } else {
+    len = min(len, opk - xs);  // (up to this token's literals: the next run reads them from their stream)
+    r.src = o + xs;
+    if (xs + 16 > raw) r.kind = 2;
+  }
+  r.len = len;
+  return r;
+}
+
+// One group's output [gs, ge) copied by the whole wavefront in 16-byte pieces, piece j on lane j % 64. A piece is
+// assembled from runs (a run lies in one token's literals, or in one stretch of its match that reads consecutive
+// bytes), three at a time: their loads go out together, then they are spliced into one register and the piece is
+// stored whole. The store may run past the group's end: those bytes belong to later groups (or the trailing
+// literals), written after this one in program order. Only a short pattern straddling a token's literals, or a
+// piece at the block's end, goes byte by byte. The token holding each piece's first byte comes from a prefix
+// maximum over the pieces where tokens start.
+__device__ inline void group_copy(uint8_t* o, const uint8_t* litb, uint32_t raw, uint32_t ka, uint32_t kb, uint32_t gs,
+                                  uint32_t ge, uint32_t opk_lane, const uint32_t* sop, const uint32_t* sL,
+                                  const uint32_t* sM, const uint32_t* sD, const uint32_t* sli, uint32_t* sown) {
+  const uint32_t lane = lane_id();
+  const bool in_group = lane >= ka && lane < kb;
+  for (uint32_t base = gs; base < ge; base += 64 * 16) {
+    sown[lane] = 0;
+    wave_fence();
+    const uint32_t jk = opk_lane <= base ? 0u : (opk_lane - base + 15u) >> 4;
+    if (in_group && jk < 64) atomicMax(&sown[jk], lane + 1);
+    wave_fence();
+    uint32_t k = scan_max(sown[lane]) - 1;
+    const uint32_t k0 = k;
+    wave_fence();
+    const uint32_t p0 = base + 16 * lane;
+    const bool live = p0 < ge;
+    const uint32_t pend = min(p0 + 16, ge);
+    uint32_t x = p0;
+    bool slow = live && p0 + 16 > raw;
+    U128 piece{0ull, 0ull};
+    while (ballot(live && !slow && x < pend)) {
+      Run r[3];
+      uint32_t off[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        off[i] = x - p0;
+        r[i].src = litb;
+        r[i].len = 0;
+        r[i].kind = 0;
+        if (live && !slow && x < pend) {
+          const uint32_t kk = min(k, 63u);
+          const uint32_t opk = sop[kk], L = sL[kk], M = sM[kk];
+          r[i] = run_at(o, litb, raw, opk, L, M, sD[kk], sli[kk], x, pend);
+          if (r[i].kind == 2) { slow = true; r[i].src = litb; r[i].len = 0; }
+          x += r[i].len;
+          if (x == opk + L + M) ++k;
+        }
+      }
+      U128 v[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) v[i] = ld128(r[i].src);
+      if (ballot((r[0].kind | r[1].kind | r[2].kind) == 1u)) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+          if (r[i].kind == 1) v[i] = pattern16(v[i], r[i].phase, r[i].period);
+      }
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        if (r[i].len) piece = splice(piece, v[i], off[i]);
+    }
+    if (live && !slow) st128(o + p0, piece);
+    if (ballot(slow)) {
+      if (slow) {  // exact, byte by byte, from the piece's first token again
+        uint32_t t2 = k0;
+        for (uint32_t y = p0; y < pend; ++y) {
+          while (t2 < 63 && sop[t2] + sL[t2] + sM[t2] <= y) ++t2;
+          const uint32_t opk = sop[t2], L = sL[t2], D = sD[t2], lik = sli[t2];
+          const uint32_t ms = opk + L;
+          uint8_t v8;
+          if (y < ms) {
+            v8 = litb[lik + (y - opk)];
+          } else {
+            const uint32_t xs = ms - D + mod_small(y - ms, D);
+            v8 = xs >= opk ? litb[lik + (xs - opk)] : o[xs];
+          }
+          o[y] = v8;
+        }
+      }
+    }
+  }
+}
+
+// Tokens go 64 at a time: a chunk may hold the end of one group and several whole groups. The fields, positions
+// and offsets of the whole chunk are computed together (scans by DPP), with the next chunk's bytes already
+// requested; then the chunk's groups are written one after another (a match reads output from before its group
+// only, so the tokens of one group copy side by side). Checks follow the sequential decoder
+// (src/codec2.h:173-202), plus the group rule.
 __global__ __launch_bounds__(64) void k_dec_lz(const Block* __restrict__ blocks, const uint8_t* __restrict__ comp,
                                                uint32_t blog, const uint8_t* __restrict__ lit,
                                                const uint8_t* __restrict__ tok, const uint8_t* __restrict__ ext,
                                                const uint8_t* __restrict__ sym, const uint8_t* __restrict__ grp,
                                                uint8_t* out, uint32_t* __restrict__ err) {
+  __shared__ uint32_t sflag[64];  // token k of the chunk starts a group
+  __shared__ uint32_t sgs[64];    // output start of the chunk's k-th group
+  __shared__ uint32_t sop[64], sL[64], sM[64], sD[64], sli[64], sown[64];
   const Geo G = geo_of(blog);
   const uint32_t b = blockIdx.x;
   const uint32_t lane = threadIdx.x;
@@ -228,75 +380,120 @@ __global__ __launch_bounds__(64) void k_dec_lz(const Block* __restrict__ blocks,
   const uint64_t nbits_total = (uint64_t)nob * 8u;
 
   uint32_t t = 0, op = 0, li = 0, ei = 0, bp = 0;
+  uint32_t g = 0, grem = 0, gs_open = 0;  // groups started so far; tokens left in the open group; its output start
   uint32_t r0 = 1, r1 = 4, r2 = 8, l0 = 1, l1 = 4, l2 = 8;
   uint32_t fail = 0;
-  for (uint32_t g = 0; g < ng && !fail; ++g) {
-    const uint32_t gn = grpb[g];
-    if (gn == 0 || t + gn > nt) { fail = E_TOKEN; break; }
-    const uint32_t gstart = op, gfirst = t, tend = t + gn;
-    while (t < tend) {
-      const uint32_t c = min(64u, tend - t);
-      const bool on = lane < c;
-      const uint32_t tv = on ? tokb[t + lane] : 0u;
-      const uint32_t sv = on ? symb[t + lane] : 0u;
-      const uint32_t fl = on && (tv >> 4) == 15u, fm = on && (tv & 15u) == 15u;
-      const uint32_t nx = 3u * fl + 2u * fm;
-      const uint32_t xq = ei + wave_scan_excl(nx);
-      bool bad = on && (uint64_t)xq + nx > ne;
-      uint32_t L = tv >> 4, M = on ? (tv & 15u) + 4u : 0u;
-      if (!bad) {
-        if (fl) L += (uint32_t)extb[xq] | (uint32_t)extb[xq + 1] << 8 | (uint32_t)extb[xq + 2] << 16;
-        if (fm) { const uint32_t q = xq + 3u * fl; M += (uint32_t)extb[q] | (uint32_t)extb[q + 1] << 8; }
-      }
-      const bool rep = sv >= 250u;
-      if (on && (rep ? sv > 252u : sv > 19u)) bad = true;
-      const uint32_t nb_ = on && !rep && sv <= 19u ? sv : 0u;
-      const uint32_t bq = bp + wave_scan_excl(nb_);
-      uint32_t D = 0;
-      if (on && !bad) {
-        if (rep) {
-          D = sv == 250u ? r0 : sv == 251u ? r1 : r2;
-        } else if ((uint64_t)bq + nb_ > nbits_total) {
-          bad = true;
-        } else {
-          uint32_t w = 0;
-          const uint32_t at = bq >> 3;
+  uint32_t tv_n = lane < nt ? tokb[lane] : 0u, sv_n = lane < nt ? symb[lane] : 0u;
+  uint32_t gw_n = lane < ng ? grpb[lane] : 0u;
+  while (t < nt) {
+    const uint32_t c = min(64u, nt - t);
+    const bool on = lane < c;
+    const uint32_t tv = on ? tv_n : 0u, sv = on ? sv_n : 0u;
+    // the groups: lane j holds group g+j's size and its end, counted from the chunk's first token
+    const bool gvalid = g + lane < ng;
+    const uint32_t gsz = gvalid ? gw_n : 0u;
+    const uint32_t gend = grem + scan_add(gsz);
+    const uint32_t gbeg = gend - gsz;
+    const bool starts = gvalid && gbeg < c;
+    const uint32_t cover = max_all(gvalid ? gend : grem);
+    if (ballot(starts && gsz == 0) || cover < c) { fail = E_TOKEN; break; }
+    sflag[lane] = 0;
+    wave_fence();
+    if (starts) sflag[gbeg] = 1;
+    wave_fence();  // (LDS handoffs between lanes: the wavefront runs them in order, the fence keeps the compiler's)
+    const uint64_t startm = ballot(starts);
+    const uint32_t nstart = popc64(startm);
+    const uint32_t first = on ? sflag[lane] : 0u;
+    const uint32_t lastend = nstart ? readlane(gend, nstart - 1) : grem;
+    const uint32_t grem2 = lastend - c;
+    const uint32_t g2 = g + nstart;
+    {  // the next chunk's bytes, requested now
+      const uint32_t tn = t + c + lane;
+      tv_n = tn < nt ? tokb[tn] : 0u;
+      sv_n = tn < nt ? symb[tn] : 0u;
+      gw_n = g2 + lane < ng ? grpb[g2 + lane] : 0u;
+    }
+    // lengths: L >= 15 has L-15 in 3 ext bytes, then M-4 >= 15 has M-19 in 2 (src/codec2.h:178-180); offsets: b
+    // raw bits for symbol b (0..19), or one of the group before's last three (250..252). One scan carries the
+    // token's ext bytes (bits 0-9), offset bits (10-20) and group start flag (21-27).
+    const uint32_t fl = on && (tv >> 4) == 15u, fm = on && (tv & 15u) == 15u;
+    const uint32_t nx = 3u * fl + 2u * fm;
+    const bool rep = sv >= 250u;
+    bool bad = on && (rep ? sv > 252u : sv > 19u);
+    const uint32_t nb_ = on && !rep && sv <= 19u ? sv : 0u;
+    const uint32_t pk = scan_add(nx | nb_ << 10 | first << 21);
+    const uint32_t xq = ei + ((pk & 1023u) - nx);
+    const uint32_t bq = bp + (((pk >> 10) & 2047u) - nb_);
+    const uint32_t rg = (pk >> 21) - (grem == 0 ? 1u : 0u);  // the token's group, from 0 in the chunk
+    const uint32_t tail = readlane(pk, 63);
+    const uint32_t ngc = readlane(rg, c - 1) + 1;
+    if (on && (uint64_t)xq + nx > ne) bad = true;
+    uint32_t L = tv >> 4, M = on ? (tv & 15u) + 4u : 0u;
+    if (!bad) {
+      if (fl) L += (uint32_t)extb[xq] | (uint32_t)extb[xq + 1] << 8 | (uint32_t)extb[xq + 2] << 16;
+      if (fm) { const uint32_t q = xq + 3u * fl; M += (uint32_t)extb[q] | (uint32_t)extb[q + 1] << 8; }
+    }
+    uint32_t D = 0;
+    if (on && !bad && !rep) {
+      if ((uint64_t)bq + nb_ > nbits_total) {
+        bad = true;
+      } else {
+        uint32_t w = 0;
+        const uint32_t at = bq >> 3;
 #pragma unroll
-          for (uint32_t k = 0; k < 4; ++k)
-            if (at + k < nob) w |= (uint32_t)bits[at + k] << (8 * k);
-          D = (1u << nb_) + ((w >> (bq & 7u)) & ((1u << nb_) - 1u));
-        }
+        for (uint32_t k = 0; k < 4; ++k)
+          if (at + k < nob) w |= (uint32_t)bits[at + k] << (8 * k);
+        D = (1u << nb_) + ((w >> (bq & 7u)) & ((1u << nb_) - 1u));
       }
-      const uint32_t tot = L + M;
-      const uint32_t opk = op + wave_scan_excl(tot);
-      const uint32_t lik = li + wave_scan_excl(L);
-      const uint32_t ms = opk + L;
-      if (on && !bad && ((uint64_t)lik + L > nl || (uint64_t)opk + tot > raw || D == 0 || D > ms)) bad = true;
-      const uint32_t src = ms - D;
-      // the group rule: a match reads output from before its group, or its own literals (src/codec2.h:1-9)
-      const bool gbad = on && !bad && src < opk && t + lane != gfirst && src + min(M, D) > gstart;
-      if (ballot(bad || gbad)) { fail = ballot(bad) ? E_TOKEN : E_GROUP; break; }
-      if (on) {
-        if (L) copy_exact(o + opk, litb + lik, L);
-        if (M) {
-          const uint8_t* sp = src >= opk ? litb + lik + (src - opk) : o + src;
-          copy_match(o + ms, sp, D, M);
-        }
-      }
-      op += wave_sum(tot);
-      li += wave_sum(L);
-      ei += wave_sum(nx);
-      bp += wave_sum(nb_);
-      for (uint32_t q = c > 3u ? c - 3u : 0u; q < c; ++q) {
-        const uint32_t v = readlane(D, q);
+    }
+    // repeated offsets, group by group: recent = the last three offsets as of the end of the group before
+    for (uint32_t q = 0; q < ngc; ++q) {
+      if (q > 0 || grem == 0) { r0 = l0; r1 = l1; r2 = l2; }  // group q starts here
+      const bool inq = on && rg == q;
+      if (inq && rep && !bad) D = sv == 250u ? r0 : sv == 251u ? r1 : r2;
+      const uint64_t qm = ballot(inq);
+      if (!qm) continue;  // (every group of the chunk holds a token of it)
+      const uint32_t qb = 64u - (uint32_t)__builtin_clzll(qm), qa = (uint32_t)__builtin_ctzll(qm);
+      for (uint32_t x = qb - qa > 3u ? qb - 3u : qa; x < qb; ++x) {
+        const uint32_t v = readlane(D, x);
         l2 = l1; l1 = l0; l0 = v;
       }
-      t += c;
     }
-    r0 = l0; r1 = l1; r2 = l2;
-    wave_fence();  // this group's output is read by the next
+    const uint32_t tot = L + M;
+    const uint32_t itot = scan_add(tot), il = scan_add(L);
+    const uint32_t opk = op + itot - tot;
+    const uint32_t lik = li + il - L;
+    const uint32_t ms = opk + L;
+    if (first) sgs[rg] = opk;
+    wave_fence();
+    const uint32_t gstart = (rg == 0 && grem > 0) ? gs_open : sgs[rg & 63u];
+    if (on && !bad && ((uint64_t)lik + L > nl || (uint64_t)opk + tot > raw || D == 0 || D > ms)) bad = true;
+    const uint32_t src = ms - D;
+    // the group rule: a match reads output from before its group, or its own literals (src/codec2.h:1-9)
+    const bool gbad = on && !bad && src < opk && !first && src + min(M, D) > gstart;
+    if (ballot(bad || gbad)) { fail = ballot(bad) ? E_TOKEN : E_GROUP; break; }
+    sop[lane] = opk; sL[lane] = L; sM[lane] = M; sD[lane] = D; sli[lane] = lik;
+    wave_fence();
+    if (grem2 && nstart) gs_open = readlane(opk, readlane(gbeg, nstart - 1));  // (the open group's first token)
+    const uint32_t op_end = op + readlane(itot, 63);
+    op = op_end;
+    li += readlane(il, 63);
+    ei += tail & 1023u;
+    bp += (tail >> 10) & 2047u;
+    t += c;
+    g = g2;
+    grem = grem2;
+    for (uint32_t q = 0; q < ngc; ++q) {
+      const uint64_t qm = ballot(on && rg == q);
+      if (!qm) continue;
+      const uint32_t ka = (uint32_t)__builtin_ctzll(qm), kb = 64u - (uint32_t)__builtin_clzll(qm);
+      const uint32_t gs = readlane(opk, ka), ge = readlane(opk + tot, kb - 1);
+      group_copy(o, litb, raw, ka, kb, gs, ge, opk, sop, sL, sM, sD, sli, sown);
+      wave_fence();  // this group's output is read by the next
+    }
+    wave_fence();
   }
-  if (!fail && (t != nt || op + (nl - li) != raw)) fail = E_TOKEN;
+  if (!fail && (g != ng || grem != 0 || op + (nl - li) != raw)) fail = E_TOKEN;
   if (fail) { if (lane == 0) atomicOr(&err[b], fail); return; }
   wave_copy(o + op, litb + li, nl - li);
 }

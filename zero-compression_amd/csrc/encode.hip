@@ -5,8 +5,8 @@
 // Per launch: nb blocks, block b's raw bytes at in + blocks[b].src (blocks[b].raw_len of them). Per-block scratch
 // is indexed by b (see EncScratch in capi.cpp). Stages:
 //   k_enc_hash      hash5 of every position, bucket counts and a scatter into buckets, counters in LDS
-//   k_enc_sort_s    buckets of <= 16 entries sorted in registers; larger ones queued
-//   k_enc_sort_m    queued buckets of <= 4096 entries: bitonic sort in LDS
+//   k_enc_sort_s    buckets of <= 16 entries sorted in registers, <= 64 across a wavefront; larger ones queued
+//   k_enc_sort_m    queued buckets of 65..4096 entries: bitonic sort in LDS
 //   k_enc_sort_l    larger buckets: a workgroup's stable LSD radix sort through global memory
 //   k_enc_match     the best match of every position among its run check and sorted neighbours
 //   k_enc_lazy      a match gives way to a better one a byte later
@@ -109,51 +109,76 @@ __global__ __launch_bounds__(1024) void k_enc_hash(const Block* __restrict__ blo
 
 // ---------------------------------------------------------------- 2. bucket sorts (the values are distinct)
 struct SortQueues {
-  uint32_t* mid;    // (block, bucket) pairs of 17..4096 entries
+  uint32_t* mid;    // (block, bucket) pairs of 65..4096 entries
   uint32_t* large;  // pairs of more than 4096 entries
   uint32_t* counts; // [0] mid, [1] large
   uint32_t cap;     // pairs each queue can hold
 };
 
+// Buckets of <= 16 entries: one thread each, a bitonic network in registers. Buckets of 17..64: the thread's
+// wavefront sorts them one after another, one entry per lane, a bitonic network over lane shuffles. Larger buckets
+// are queued for k_enc_sort_m (<= 4096) or k_enc_sort_l.
 __global__ __launch_bounds__(256) void k_enc_sort_s(const Block* __restrict__ blocks, uint32_t nb, uint32_t blog,
                                                     uint32_t* __restrict__ srt, const uint32_t* __restrict__ bstart,
                                                     SortQueues q) {
   const uint32_t HB = 1u << hbits_of(blog);
   const uint64_t gid = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   const uint32_t b = (uint32_t)(gid / HB), h = (uint32_t)(gid % HB);
-  if (b >= nb) return;
-  const uint32_t* bs = bstart + (size_t)b * (HB + 1);
-  const uint32_t lo = bs[h], k = bs[h + 1] - lo;
-  if (k < 2) return;
-  if (k > 16) {
+  const bool valid = b < nb;
+  uint32_t lo = 0, k = 0;
+  if (valid) {
+    const uint32_t* bs = bstart + (size_t)b * (HB + 1);
+    lo = bs[h];
+    k = bs[h + 1] - lo;
+  }
+  uint32_t* s = srt + ((size_t)b << blog) + lo;
+  if (k > 64) {
     const uint32_t which = k > 4096 ? 1 : 0;
     const uint32_t at = atomicAdd(&q.counts[which], 1u);
     if (at < q.cap) { uint32_t* dst = which ? q.large : q.mid; dst[2 * at] = b; dst[2 * at + 1] = h; }
-    return;
-  }
-  uint32_t* s = srt + ((size_t)b << blog) + lo;
-  uint32_t v[16];
+  } else if (k >= 2 && k <= 16) {
+    uint32_t v[16];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) v[i] = (uint32_t)i < k ? s[i] : 0xFFFFFFFFu;
+    for (int i = 0; i < 16; ++i) v[i] = (uint32_t)i < k ? s[i] : 0xFFFFFFFFu;
 #pragma unroll
-  for (int kk = 2; kk <= 16; kk <<= 1)
+    for (int kk = 2; kk <= 16; kk <<= 1)
 #pragma unroll
-    for (int j = kk >> 1; j > 0; j >>= 1)
+      for (int j = kk >> 1; j > 0; j >>= 1)
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const int l = i ^ j;
-        if (l > i) {
-          const bool asc = (i & kk) == 0;
-          const uint32_t a = v[i], c = v[l];
-          if ((a > c) == asc) { v[i] = c; v[l] = a; }
+        for (int i = 0; i < 16; ++i) {
+          const int l = i ^ j;
+          if (l > i) {
+            const bool asc = (i & kk) == 0;
+            const uint32_t a = v[i], c = v[l];
+            if ((a > c) == asc) { v[i] = c; v[l] = a; }
+          }
         }
-      }
 #pragma unroll
-  for (int i = 0; i < 16; ++i)
-    if ((uint32_t)i < k) s[i] = v[i];
+    for (int i = 0; i < 16; ++i)
+      if ((uint32_t)i < k) s[i] = v[i];
+  }
+  uint64_t m = ballot(k > 16 && k <= 64);
+  const uint32_t lane = lane_id();
+  while (m) {
+    const uint32_t who = (uint32_t)__builtin_ctzll(m);
+    m &= m - 1;
+    const uint32_t kw = readlane(k, who);
+    uint32_t* sw = (uint32_t*)(((uint64_t)readlane((uint32_t)((uint64_t)s >> 32), who) << 32) |
+                               readlane((uint32_t)(uint64_t)s, who));
+    uint32_t v = lane < kw ? sw[lane] : 0xFFFFFFFFu;
+#pragma unroll
+    for (uint32_t kk = 2; kk <= 64; kk <<= 1)
+#pragma unroll
+      for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
+        const uint32_t o = (uint32_t)__shfl_xor((int)v, (int)j, 64);
+        const bool keep_min = ((lane & j) == 0) == ((lane & kk) == 0);
+        v = keep_min ? min(v, o) : max(v, o);
+      }
+    if (lane < kw) sw[lane] = v;
+  }
 }
 
-// queued buckets of 17..4096 entries: one workgroup per bucket, bitonic sort in LDS (padded to a power of two)
+// queued buckets of 65..4096 entries: one workgroup per bucket, bitonic sort in LDS (padded to a power of two)
 __global__ __launch_bounds__(256) void k_enc_sort_m(uint32_t blog, uint32_t* __restrict__ srt,
                                                     const uint32_t* __restrict__ bstart, SortQueues q) {
   __shared__ uint32_t a[4096];
@@ -164,7 +189,7 @@ __global__ __launch_bounds__(256) void k_enc_sort_m(uint32_t blog, uint32_t* __r
     const uint32_t* bs = bstart + (size_t)b * (HB + 1);
     const uint32_t lo = bs[h], k = bs[h + 1] - lo;
     uint32_t* s = srt + ((size_t)b << blog) + lo;
-    uint32_t np2 = 32;
+    uint32_t np2 = 128;
     while (np2 < k) np2 <<= 1;
     for (uint32_t i = threadIdx.x; i < np2; i += 256) a[i] = i < k ? s[i] : 0xFFFFFFFFu;
     __syncthreads();

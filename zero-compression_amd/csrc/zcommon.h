@@ -80,7 +80,35 @@ __device__ inline uint32_t wave_min(uint32_t v) {
   for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor(v, o, 64));
   return v;
 }
-__device__ inline uint32_t readlane(uint32_t v, uint32_t lane) { return __shfl(v, (int)lane, 64); }
+// v in `lane` (the lane index must be the same in every lane: v_readlane)
+__device__ inline uint32_t readlane(uint32_t v, uint32_t lane) { return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane); }
+
+// DPP forms (gfx9 data-parallel primitives: row shifts inside 16-lane rows, broadcasts of lanes 15 and 31): an
+// inclusive scan is six VALU operations with no LDS traffic. Lanes a DPP source does not reach read 0.
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ inline uint32_t dpp0(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xF, true);
+}
+__device__ inline uint32_t scan_add(uint32_t v) {  // inclusive
+  v += dpp0<0x111>(v);       // row_shr:1
+  v += dpp0<0x112>(v);       // row_shr:2
+  v += dpp0<0x114>(v);       // row_shr:4
+  v += dpp0<0x118>(v);       // row_shr:8
+  v += dpp0<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp0<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+  return v;
+}
+__device__ inline uint32_t scan_max(uint32_t v) {  // inclusive (values >= 0: a missing source reads 0)
+  v = max(v, dpp0<0x111>(v));
+  v = max(v, dpp0<0x112>(v));
+  v = max(v, dpp0<0x114>(v));
+  v = max(v, dpp0<0x118>(v));
+  v = max(v, dpp0<0x142, 0xA>(v));
+  v = max(v, dpp0<0x143, 0xC>(v));
+  return v;
+}
+__device__ inline uint32_t sum_all(uint32_t v) { return __builtin_amdgcn_readlane(scan_add(v), 63); }
+__device__ inline uint32_t max_all(uint32_t v) { return __builtin_amdgcn_readlane(scan_max(v), 63); }
 
 // ordering of global-memory writes and later reads among the lanes of one wavefront (a compiler barrier; the
 // hardware keeps one wavefront's vector memory operations in order)

@@ -95,8 +95,8 @@ def load_library(path=LIB_PATH):
         "zgpu_crc32_blocks": (i32, [vp, vp, vp, u32, i32, vp, vp]),
         "zgpu_encode_scratch_bytes": (ctypes.c_size_t, [u32, u32]),
         "zgpu_decode_scratch_bytes": (ctypes.c_size_t, [u32, u32]),
-        "zgpu_compress": (i32, [vp, vp, u64, u32, i32, vp, vp]),
-        "zgpu_decompress": (i32, [vp, vp, vp, vp, vp, u32, u32, vp, vp]),
+        "zgpu_compress": (i32, [vp, vp, u64, u32, i32, vp, vp, vp]),
+        "zgpu_decompress": (i32, [vp, vp, vp, vp, vp, u32, u32, vp, vp, vp]),
         "zgpu_crc32": (i32, [vp, vp, u64, u32, vp]),
         "zgpu_set_profile": (i32, [vp, i32]),
         "zgpu_stage_times": (i32, [vp, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_float), i32]),
@@ -191,7 +191,7 @@ class Gpu:
             return []
         out = np.empty(nb * B, dtype=np.uint8)
         lens = np.empty(nb, dtype=np.uint32)
-        self._check(self.lib.zgpu_compress(self.h, _ptr(a), n, block_log, level, _ptr(out), _ptr(lens)))
+        self._check(self.lib.zgpu_compress(self.h, _ptr(a), n, block_log, level, _ptr(out), _ptr(lens), None))
         res = []
         for b in range(nb):
             if lens[b] == STORED:
@@ -211,8 +211,10 @@ class Gpu:
         rlen = np.array([r for _, r, _ in blocks], dtype=np.uint32)
         out = np.empty(max(1, int(rlen.sum(dtype=np.uint64))), dtype=np.uint8)
         err = np.zeros(nb, dtype=np.uint32)
+        crc = np.zeros(nb, dtype=np.uint32)
         rc = self.lib.zgpu_decompress(self.h, _ptr(comp), _ptr(clen), _ptr(meth), _ptr(rlen), nb, block_log,
-                                      _ptr(out), _ptr(err))
+                                      _ptr(out), _ptr(err), _ptr(crc))
+        self.last_crc = crc
         self.last_err = err
         self._check(rc)
         return out[: int(rlen.sum(dtype=np.uint64))].tobytes()
