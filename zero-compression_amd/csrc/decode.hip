@@ -148,13 +148,22 @@ __global__ __launch_bounds__(64) void k_dec_streams(const Block* __restrict__ bl
   }
   __syncthreads();
   // 64 lanes decode side by side: lane j holds symbols j, j+64, ... A step needs 0, 1 or 2 more bytes of the lane
-  // (x >= 2^11 after the step: 2^15 <= x < 2^23 takes one, x < 2^15 two), read with one 2-byte load from the
-  // lane's position (cached), so the loop has no branch; a lane that reads past its end is damaged.
+  // (x >= 2^11 after the step: 2^15 <= x < 2^23 takes one, x < 2^15 two). Each lane keeps a window w of nw <= 8 of
+  // its next bytes and the 8 after them, requested when w was last refilled, so a step never waits on a load it
+  // just issued. Reads stop at the payload's end; a lane that consumes more than its length is damaged.
   const uint8_t* lp = body + lstart[lane];
   const uint32_t lnl = lstart[lane + 1] - lstart[lane];
   const uint8_t* lim = p + blk.comp_len;
   uint32_t x = ld32u(lp);
-  uint32_t pos = 4;
+  const uint8_t* rd = lp + 4;
+  auto fetch8 = [&]() {
+    const uint64_t v = load8n(rd, rd < lim ? (uint32_t)min((size_t)8, (size_t)(lim - rd)) : 0u);
+    rd += 8;
+    return v;
+  };
+  uint64_t w = fetch8();
+  uint64_t nxt = fetch8();
+  uint32_t nw = 8, used = 4;
   for (uint32_t i = lane; i < n; i += 64) {
     const uint32_t sl = x & 4095u;
     const uint32_t sy = slot[sl];
@@ -162,20 +171,20 @@ __global__ __launch_bounds__(64) void k_dec_streams(const Block* __restrict__ bl
     dest[i] = (uint8_t)sy;
     x = (e & 0xFFFFu) * (x >> 12) + sl - (e >> 16);
     const uint32_t need = (x < (1u << 15) ? 1u : 0u) + (x < (1u << 23) ? 1u : 0u);
-    const uint8_t* q = lp + pos;
-    uint32_t b01;
-    if (q + 2 <= lim) {
-      uint16_t h;
-      __builtin_memcpy(&h, q, 2);
-      b01 = h;
-    } else {
-      b01 = (q < lim ? (uint32_t)q[0] : 0u) | (q + 1 < lim ? (uint32_t)q[1] << 8 : 0u);
-    }
-    const uint32_t b0 = b01 & 0xFFu, b1 = b01 >> 8;
+    const uint64_t cat = nw >= 2 ? w : nw == 1 ? (w & 0xFFull) | (nxt << 8) : nxt;  // the next two bytes
+    const uint32_t b0 = (uint32_t)cat & 0xFFu, b1 = (uint32_t)(cat >> 8) & 0xFFu;
     x = need == 0 ? x : need == 1 ? (x << 8) | b0 : (x << 16) | (b0 << 8) | b1;
-    pos += need;
+    used += need;
+    if (need > nw) {  // w is used up: go on in nxt and request the 8 bytes after it
+      w = nxt >> (8u * (need - nw));
+      nw += 8u - need;
+      nxt = fetch8();
+    } else {
+      w >>= 8u * need;
+      nw -= need;
+    }
   }
-  if (ballot(pos > lnl) && lane == 0) atomicOr(&err[b], E_LANE);
+  if (ballot(used > lnl) && lane == 0) atomicOr(&err[b], E_LANE);
 }
 
 // ---------------------------------------------------------------- the LZ replay: one wavefront per block
