@@ -57,6 +57,22 @@ def cpu_baseline(data, bl, depth, budget_s=12.0):
     }
 
 
+def traffic_of(kernel, n, level, bl):
+    """HBM bytes per launch of `kernel` on this workload, from the committed counter summary (profiles/traffic.json:
+    rocprofv3 FETCH_SIZE and WRITE_SIZE passes, each scaled by what the counter reads for k_copy of known size), or
+    None when the summary is for another workload or kernel."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        t = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    w = t.get("workload", {})
+    if (w.get("input_bytes"), w.get("level"), w.get("block_log")) != (n, level, bl):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    return None if k is None else k["hbm_bytes_per_launch"]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,12 +183,21 @@ def main():
     err = d_err.download(dtype=np.uint32)
     ok = bool((err == 0).all()) and np.array_equal(d_out.download(), data)
     log(f"decode {dec_ms:.3f} ms/step = {n / dec_ms / 1e6:.1f} GB/s, round trip {'identical' if ok else 'MISMATCH'}")
+    # a plain device copy of the same size (k_copy): the bandwidth this GPU gives a streaming kernel
+    gpu.copy(d_out.ptr, d_in.ptr, n)
+    gpu.record(ev0)
+    for _ in range(a.steps):
+        gpu.copy(d_out.ptr, d_in.ptr, n)
+    gpu.record(ev1)
+    gpu.sync()
+    copy_ms = gpu.elapsed_ms(ev0, ev1) / a.steps
+    log(f"copy {n >> 20} MiB: {copy_ms:.3f} ms = {2 * n / copy_ms / 1e6:.1f} GB/s (read + write)")
     dom = max(stage_avg, key=stage_avg.get)
     alg = comp_total + n
     roof = {"bound": "hbm", "kernel": f"k_dec_{dom}", "achieved": round(alg / (stage_avg[dom] * 1e-3) / 1e9, 1),
             "peak": HBM_PEAK / 1e9, "unit": "GB/s"}
     roof["frac"] = round(roof["achieved"] * 1e9 / HBM_PEAK, 4)
-    roof["traffic"] = None
+    roof["traffic"] = traffic_of(f"k_dec_{dom}", n, level, bl)
     roof["whole_decode_frac"] = round(alg / (dec_ms * 1e-3) / HBM_PEAK, 4)
 
     # ---- random 1 GiB: every block stored
@@ -240,6 +265,7 @@ def main():
                    "decompress_GBps": round(rnd.size / (r_dec_ms * 1e-3) / 1e9, 3),
                    "stored_blocks": r_stored, "blocks": nr},
         "decode_stage_ms": {k: round(v, 4) for k, v in stage_avg.items()},
+        "copy_GBps": round(2 * n / (copy_ms * 1e-3) / 1e9, 1),
         "encode_stage_ms_per_GiB": {k: round(v, 3) for k, v in enc_stages},
         "roofline": roof,
         "cpu_baseline": cpu,

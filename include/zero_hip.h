@@ -76,6 +76,9 @@ int zgpu_free_host(zgpu_t* g, void* hptr);
 int zgpu_upload(zgpu_t* g, void* dptr, const void* host, size_t bytes);   /* synchronous */
 int zgpu_download(zgpu_t* g, void* host, const void* dptr, size_t bytes); /* synchronous */
 int zgpu_fill(zgpu_t* g, void* dptr, uint8_t value, size_t bytes, zgpu_stream_t s); /* a kernel, graph-safe */
+/* device-to-device copy on the stream by a plain 16-byte-per-lane kernel (k_copy) when both pointers are 16-byte
+   aligned: the codec's measured bandwidth ceiling and the calibration kernel for HBM byte counters */
+int zgpu_copy(zgpu_t* g, void* dst, const void* src, size_t bytes, zgpu_stream_t s);
 
 /* ---- streams and events (timing on the stream the kernels run on) */
 int zgpu_stream_create(zgpu_t* g, zgpu_stream_t* out);

@@ -98,6 +98,12 @@ __global__ void k_fill(uint8_t* p, uint8_t v, size_t n) {
   for (size_t k = i; k < n; k += stride) p[k] = v;
 }
 
+// a plain copy, 16 bytes per lane per step (both pointers 16-byte aligned, n a multiple of 16; the host does the rest)
+__global__ __launch_bounds__(256) void k_copy(uint4* __restrict__ d, const uint4* __restrict__ s, size_t n16) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n16; k += stride) d[k] = s[k];
+}
+
 // ---------------------------------------------------------------- scratch layouts
 namespace {
 inline size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -260,6 +266,22 @@ int zgpu_fill(zgpu_t* g, void* dptr, uint8_t value, size_t bytes, zgpu_stream_t 
   const size_t groups = std::min<size_t>((bytes + 255) / 256, 4096);
   k_fill<<<(unsigned)groups, 256, 0, st>>>((uint8_t*)dptr, value, bytes);
   HIPCHK(g, hipGetLastError());
+  return ZGPU_OK;
+}
+
+int zgpu_copy(zgpu_t* g, void* dst, const void* src, size_t bytes, zgpu_stream_t s) {
+  if (!bytes) return ZGPU_OK;
+  hipStream_t st = s ? (hipStream_t)s : g->stream;
+  const bool aligned = (((uintptr_t)dst | (uintptr_t)src) & 15u) == 0;
+  const size_t n16 = aligned ? bytes / 16 : 0;
+  if (n16) {
+    const size_t groups = std::min<size_t>((n16 + 255) / 256, (size_t)g->props.cu_count * 8);
+    k_copy<<<(unsigned)groups, 256, 0, st>>>((uint4*)dst, (const uint4*)src, n16);
+    HIPCHK(g, hipGetLastError());
+  }
+  if (bytes > n16 * 16)
+    HIPCHK(g, hipMemcpyAsync((uint8_t*)dst + n16 * 16, (const uint8_t*)src + n16 * 16, bytes - n16 * 16,
+                             hipMemcpyDeviceToDevice, st));
   return ZGPU_OK;
 }
 

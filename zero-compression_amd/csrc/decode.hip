@@ -226,124 +226,95 @@ __device__ inline U128 pattern16(U128 e, uint32_t ph, uint32_t D) {
   return splice(shr128(e, ph), shr128(e, 16u - D), 16u - ph);
 }
 
-// where the bytes from output offset x on (token k) come from: a run of `len` bytes read at `src` (kind 0), a short
-// pattern (kind 1: src is its period, `phase` the period byte at x), or the exact path (kind 2)
-struct Run {
-  const uint8_t* src;
-  uint32_t len, kind, phase, period;
-};
-__device__ inline Run run_at(uint8_t* o, const uint8_t* litb, uint32_t raw, uint32_t opk, uint32_t L, uint32_t M,
-                             uint32_t D, uint32_t lik, uint32_t x, uint32_t pend) {
-  const uint32_t ms = opk + L, me = ms + M;
-  Run r;
-  r.kind = 0;
-  r.phase = 0;
-  r.period = D;
-  if (x < ms) {
-    r.len = min(pend, ms) - x;
-    r.src = litb + lik + (x - opk);
-    return r;
-  }
-  uint32_t len = min(pend, me) - x;
-  uint32_t xs = x - D;  // (D >= M: the source of byte x)
-  if (D < M) {
-    const uint32_t ph = mod_small(x - ms, D);
-    if (D < 16) {
-      const uint32_t src = ms - D;
-      r.kind = src >= opk || ms <= opk ? 1u : 2u;
-      r.phase = ph;
-      r.len = len;
-      r.src = src >= opk ? litb + lik + (src - opk) : o + src;
-      if (r.kind == 1 && src < opk && src + 16 > raw) r.kind = 2;
-      return r;
-    }
-    xs = ms - D + ph;
-    len = min(len, D - ph);
-  }
-  if (xs >= opk) {
-    r.src = litb + lik + (xs - opk);
-  } else {
-    len = min(len, opk - xs);  // (up to this token's literals: the next run reads them from their stream)
-    r.src = o + xs;
-    if (xs + 16 > raw) r.kind = 2;
-  }
-  r.len = len;
-  return r;
+// q = a / d, r = a % d for a < 2^21, d >= 1 (quotient by the hardware reciprocal, corrected by one either way)
+__device__ inline void divmod_small(uint32_t a, uint32_t d, uint32_t& q, uint32_t& r) {
+  uint32_t qq = (uint32_t)((float)a * __builtin_amdgcn_rcpf((float)d));
+  int32_t rr = (int32_t)(a - qq * d);
+  if (rr < 0) { rr += (int32_t)d; --qq; }
+  if (rr >= (int32_t)d) { rr -= (int32_t)d; ++qq; }
+  q = qq;
+  r = (uint32_t)rr;
+}
+// stores the first len (0..15) bytes of v at d
+__device__ inline void st_partial(uint8_t* d, U128 v, uint32_t len) {
+  uint64_t w = v.lo;
+  if (len & 8u) { __builtin_memcpy(d, &w, 8); w = v.hi; d += 8; }
+  if (len & 4u) { const uint32_t x = (uint32_t)w; __builtin_memcpy(d, &x, 4); w >>= 32; d += 4; }
+  if (len & 2u) { const uint16_t x = (uint16_t)w; __builtin_memcpy(d, &x, 2); w >>= 16; d += 2; }
+  if (len & 1u) *d = (uint8_t)w;
 }
 
-// One group's output [gs, ge) copied by the whole wavefront in 16-byte pieces, piece j on lane j % 64. A piece is
-// assembled from runs (a run lies in one token's literals, or in one stretch of its match that reads consecutive
-// bytes), three at a time: their loads go out together, then they are spliced into one register and the piece is
-// stored whole. The store may run past the group's end: those bytes belong to later groups (or the trailing
-// literals), written after this one in program order. Only a short pattern straddling a token's literals, or a
-// piece at the block's end, goes byte by byte. The token holding each piece's first byte comes from a prefix
-// maximum over the pieces where tokens start.
-__device__ inline void group_copy(uint8_t* o, const uint8_t* litb, uint32_t raw, uint32_t ka, uint32_t kb, uint32_t gs,
-                                  uint32_t ge, uint32_t opk_lane, const uint32_t* sop, const uint32_t* sL,
-                                  const uint32_t* sM, const uint32_t* sD, const uint32_t* sli, uint32_t* sown) {
+// Chunks of a token: its literals in 16-byte chunks, then its match in chunks that each read one stretch of
+// consecutive source bytes. A match whose offset D is at least its length M reads [ms - D, me - D) (ms, me: the
+// match's output range); one with 16 <= D < M repeats the period [ms - D, ms), each repetition cut into
+// ceil(D/16) chunks; one with D < 16 is written 16 bytes at a time as the period pattern.
+__device__ inline uint32_t chunk_count(uint32_t L, uint32_t M, uint32_t D) {
+  const uint32_t nlit = (L + 15u) >> 4;
+  if (D < 16u || D >= M) return nlit + ((M + 15u) >> 4);
+  uint32_t reps, rem;
+  divmod_small(M, D, reps, rem);
+  return nlit + reps * ((D + 15u) >> 4) + ((rem + 15u) >> 4);
+}
+
+// One group's chunks [cb, ce) copied by the whole wavefront, chunk cb + 64r + j on lane j in round r. Lane k < 64
+// holds token k of the chunk of tokens (its first chunk ccs_lane, its group membership in_group); the token that
+// holds a lane's chunk is the prefix maximum over the chunk slots where tokens start. A chunk loads its 16 source
+// bytes at once (from the literal stream when they are the token's own literals, else from output written by an
+// earlier group), and stores exactly its length. Sources that straddle the token's first output byte take a second
+// load; a 16-byte read of output past the block's end goes byte by byte.
+__device__ inline void group_copy(uint8_t* o, const uint8_t* litb, uint32_t raw, bool in_group, uint32_t ccs_lane,
+                                  uint32_t cb, uint32_t ce, const uint32_t* sop, const uint32_t* sL,
+                                  const uint32_t* sM, const uint32_t* sD, const uint32_t* sli, const uint32_t* scs,
+                                  uint32_t* sown) {
   const uint32_t lane = lane_id();
-  const bool in_group = lane >= ka && lane < kb;
-  for (uint32_t base = gs; base < ge; base += 64 * 16) {
+  for (uint32_t c0 = cb; c0 < ce; c0 += 64) {
     sown[lane] = 0;
     wave_fence();
-    const uint32_t jk = opk_lane <= base ? 0u : (opk_lane - base + 15u) >> 4;
+    const uint32_t jk = ccs_lane <= c0 ? 0u : ccs_lane - c0;
     if (in_group && jk < 64) atomicMax(&sown[jk], lane + 1);
     wave_fence();
-    uint32_t k = scan_max(sown[lane]) - 1;
-    const uint32_t k0 = k;
+    const uint32_t k = min(scan_max(sown[lane]), 64u) - 1u;
     wave_fence();
-    const uint32_t p0 = base + 16 * lane;
-    const bool live = p0 < ge;
-    const uint32_t pend = min(p0 + 16, ge);
-    uint32_t x = p0;
-    bool slow = live && p0 + 16 > raw;
-    U128 piece{0ull, 0ull};
-    while (ballot(live && !slow && x < pend)) {
-      Run r[3];
-      uint32_t off[3];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        off[i] = x - p0;
-        r[i].src = litb;
-        r[i].len = 0;
-        r[i].kind = 0;
-        if (live && !slow && x < pend) {
-          const uint32_t kk = min(k, 63u);
-          const uint32_t opk = sop[kk], L = sL[kk], M = sM[kk];
-          r[i] = run_at(o, litb, raw, opk, L, M, sD[kk], sli[kk], x, pend);
-          if (r[i].kind == 2) { slow = true; r[i].src = litb; r[i].len = 0; }
-          x += r[i].len;
-          if (x == opk + L + M) ++k;
-        }
-      }
-      U128 v[3];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) v[i] = ld128(r[i].src);
-      if (ballot((r[0].kind | r[1].kind | r[2].kind) == 1u)) {
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-          if (r[i].kind == 1) v[i] = pattern16(v[i], r[i].phase, r[i].period);
-      }
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-        if (r[i].len) piece = splice(piece, v[i], off[i]);
+    const uint32_t cj = c0 + lane;
+    const bool live = cj < ce;
+    const uint32_t kk = live ? k : 0u;
+    const uint32_t opk = sop[kk], L = sL[kk], M = sM[kk], D = sD[kk], lik = sli[kk], cs = scs[kk];
+    const uint32_t w = cj - cs;
+    const uint32_t nlit = (L + 15u) >> 4;
+    const uint32_t ms = opk + L, me = ms + M;
+    const bool litc = w < nlit;
+    const bool pat = !litc && D < 16u;
+    uint32_t rep = 0, within = litc ? w : w - nlit;
+    if (ballot(live && !litc && D >= 16u && D < M)) {
+      if (!litc && D >= 16u && D < M) divmod_small(within, (D + 15u) >> 4, rep, within);
     }
-    if (live && !slow) st128(o + p0, piece);
+    const uint32_t base = litc ? opk : ms + rep * D;
+    const uint32_t y0 = base + 16u * within;
+    const uint32_t cend = litc ? ms : pat ? me : min(me, base + D);
+    const uint32_t len = live ? min(16u, cend - y0) : 0u;
+    const uint32_t s0 = litc ? y0 : pat ? ms - D : y0 - (rep + 1u) * D;
+    const uint32_t need = pat ? D : len;
+    const bool from_out = s0 < opk;
+    const bool slow = live && from_out && s0 + 16u > raw;
+    const bool straddle = live && !slow && from_out && s0 + need > opk;
+    const uint8_t* a1 = from_out ? o + s0 : litb + lik + (s0 - opk);
+    U128 v = ld128(live && !slow ? a1 : litb);
+    if (ballot(straddle)) {
+      if (straddle) v = splice(v, ld128(litb + lik), opk - s0);
+    }
+    if (ballot(live && !slow && pat)) {
+      if (live && !slow && pat) v = pattern16(v, mod_small(16u * within, D), D);
+    }
+    if (live && !slow) {
+      if (len == 16u) st128(o + y0, v);
+      else st_partial(o + y0, v, len);
+    }
     if (ballot(slow)) {
-      if (slow) {  // exact, byte by byte, from the piece's first token again
-        uint32_t t2 = k0;
-        for (uint32_t y = p0; y < pend; ++y) {
-          while (t2 < 63 && sop[t2] + sL[t2] + sM[t2] <= y) ++t2;
-          const uint32_t opk = sop[t2], L = sL[t2], D = sD[t2], lik = sli[t2];
-          const uint32_t ms = opk + L;
-          uint8_t v8;
-          if (y < ms) {
-            v8 = litb[lik + (y - opk)];
-          } else {
-            const uint32_t xs = ms - D + mod_small(y - ms, D);
-            v8 = xs >= opk ? litb[lik + (xs - opk)] : o[xs];
-          }
-          o[y] = v8;
+      if (slow) {  // exact, byte by byte
+        const uint32_t ph = pat ? mod_small(16u * within, D) : 0u;
+        for (uint32_t i = 0; i < len; ++i) {
+          const uint32_t si = pat ? s0 + mod_small(ph + i, D) : s0 + i;
+          o[y0 + i] = si < opk ? o[si] : litb[lik + (si - opk)];
         }
       }
     }
@@ -362,7 +333,7 @@ __global__ __launch_bounds__(64) void k_dec_lz(const Block* __restrict__ blocks,
                                                uint8_t* out, uint32_t* __restrict__ err) {
   __shared__ uint32_t sflag[64];  // token k of the chunk starts a group
   __shared__ uint32_t sgs[64];    // output start of the chunk's k-th group
-  __shared__ uint32_t sop[64], sL[64], sM[64], sD[64], sli[64], sown[64];
+  __shared__ uint32_t sop[64], sL[64], sM[64], sD[64], sli[64], scs[64], sown[64];
   const Geo G = geo_of(blog);
   const uint32_t b = blockIdx.x;
   const uint32_t lane = threadIdx.x;
@@ -481,7 +452,9 @@ __global__ __launch_bounds__(64) void k_dec_lz(const Block* __restrict__ blocks,
     // the group rule: a match reads output from before its group, or its own literals (src/codec2.h:1-9)
     const bool gbad = on && !bad && src < opk && !first && src + min(M, D) > gstart;
     if (ballot(bad || gbad)) { fail = ballot(bad) ? E_TOKEN : E_GROUP; break; }
-    sop[lane] = opk; sL[lane] = L; sM[lane] = M; sD[lane] = D; sli[lane] = lik;
+    const uint32_t cnt = on ? chunk_count(L, M, D) : 0u;
+    const uint32_t cinc = scan_add(cnt), ccs = cinc - cnt;
+    sop[lane] = opk; sL[lane] = L; sM[lane] = M; sD[lane] = D; sli[lane] = lik; scs[lane] = ccs;
     wave_fence();
     if (grem2 && nstart) gs_open = readlane(opk, readlane(gbeg, nstart - 1));  // (the open group's first token)
     const uint32_t op_end = op + readlane(itot, 63);
@@ -496,8 +469,8 @@ __global__ __launch_bounds__(64) void k_dec_lz(const Block* __restrict__ blocks,
       const uint64_t qm = ballot(on && rg == q);
       if (!qm) continue;
       const uint32_t ka = (uint32_t)__builtin_ctzll(qm), kb = 64u - (uint32_t)__builtin_clzll(qm);
-      const uint32_t gs = readlane(opk, ka), ge = readlane(opk + tot, kb - 1);
-      group_copy(o, litb, raw, ka, kb, gs, ge, opk, sop, sL, sM, sD, sli, sown);
+      group_copy(o, litb, raw, on && rg == q, ccs, readlane(ccs, ka), readlane(cinc, kb - 1), sop, sL, sM, sD, sli,
+                 scs, sown);
       wave_fence();  // this group's output is read by the next
     }
     wave_fence();

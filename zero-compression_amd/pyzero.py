@@ -26,7 +26,7 @@ STORED = 0xFFFFFFFF
 # every symbol include/zero_hip.h declares (tests check the library exports them all)
 EXPORTS = (
     "zgpu_open", "zgpu_close", "zgpu_props", "zgpu_last_error", "zgpu_sync",
-    "zgpu_alloc", "zgpu_free", "zgpu_alloc_host", "zgpu_free_host", "zgpu_upload", "zgpu_download", "zgpu_fill",
+    "zgpu_alloc", "zgpu_free", "zgpu_alloc_host", "zgpu_free_host", "zgpu_upload", "zgpu_download", "zgpu_fill", "zgpu_copy",
     "zgpu_stream_create", "zgpu_stream_destroy", "zgpu_stream_sync",
     "zgpu_event_create", "zgpu_event_destroy", "zgpu_event_record", "zgpu_event_elapsed_ms",
     "zgpu_encode_blocks", "zgpu_decode_blocks", "zgpu_crc32_blocks",
@@ -85,6 +85,7 @@ def load_library(path=LIB_PATH):
         "zgpu_alloc_host": (i32, [vp, ctypes.c_size_t, ctypes.POINTER(vp)]), "zgpu_free_host": (i32, [vp, vp]),
         "zgpu_upload": (i32, [vp, vp, vp, ctypes.c_size_t]), "zgpu_download": (i32, [vp, vp, vp, ctypes.c_size_t]),
         "zgpu_fill": (i32, [vp, vp, ctypes.c_uint8, ctypes.c_size_t, vp]),
+        "zgpu_copy": (i32, [vp, vp, vp, ctypes.c_size_t, vp]),
         "zgpu_stream_create": (i32, [vp, ctypes.POINTER(vp)]), "zgpu_stream_destroy": (i32, [vp, vp]),
         "zgpu_stream_sync": (i32, [vp, vp]),
         "zgpu_event_create": (i32, [vp, ctypes.POINTER(vp)]), "zgpu_event_destroy": (i32, [vp, vp]),
@@ -244,6 +245,10 @@ class Gpu:
 
     def sync(self):
         self._check(self.lib.zgpu_sync(self.h))
+
+    def copy(self, dst, src, nbytes, stream=None):
+        """device-to-device copy on the codec's stream (a plain kernel when both addresses are 16-byte aligned)"""
+        self._check(self.lib.zgpu_copy(self.h, dst, src, nbytes, stream))
 
     def event(self):
         e = vp()
