@@ -85,7 +85,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (blocks would shard across GPUs with one "
-              "all-gather of block lengths; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+              "all-gather of block lengths). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     import pyzero
 

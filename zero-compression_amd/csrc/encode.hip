@@ -3,7 +3,7 @@
 // exactly the nearest earlier positions with its 27-bit hash key, whatever order the scatter's atomics produced.
 //
 // Per launch: nb blocks, block b's raw bytes at in + blocks[b].src (blocks[b].raw_len of them). Per-block scratch
-// is indexed by b (see EncScratch in capi.cpp). Stages:
+// is indexed by b (EncLayout in capi.hip). Stages:
 //   k_enc_hash      hash5 of every position, bucket counts and a scatter into buckets, counters in LDS
 //   k_enc_sort_s    buckets of <= 16 entries sorted in registers, <= 64 across a wavefront; larger ones queued
 //   k_enc_sort_m    queued buckets of 65..4096 entries: bitonic sort in LDS
