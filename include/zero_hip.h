@@ -123,6 +123,39 @@ int zgpu_decompress(zgpu_t* g, const uint8_t* comp, const uint32_t* comp_len, co
 /* crc[b] of the nb blocks of 2^block_log in data[0..n) (host buffer) */
 int zgpu_crc32(zgpu_t* g, const uint8_t* data, uint64_t n, uint32_t block_log, uint32_t* crc);
 
+/* ---- the batched form: the blocks of many files, of uneven lengths, planned together (no reference counterpart:
+ * the reference launches its stages per 16 MiB batch of full blocks, src/gpu_codec.h:153-229, and assumes every
+ * block but the last is full, :160). A block of r bytes runs in a slab of 2^c bytes, c = max(10, ceil(log2 r)) up
+ * to block_log; each size class runs the plain path's kernels on its own slabs, classes spread over `streams`
+ * streams, and every payload is byte-identical to the plain path's (the hash keys keep block_log's width). */
+typedef struct zgpu_plan zgpu_plan_t;
+/* nb blocks: block b's raw bytes are raw_len[b] (<= 2^block_log) bytes at offset src[b] of the input buffer (host
+ * arrays, copied). streams 1..8. */
+int zgpu_plan_create(zgpu_t* g, uint32_t nb, const uint64_t* src, const uint32_t* raw_len, uint32_t block_log,
+                     int streams, zgpu_plan_t** out);
+void zgpu_plan_destroy(zgpu_plan_t* p);
+/* bytes of the output slabs zgpu_plan_encode writes; slab_off[b] (host, nb) = where block b's payload goes */
+uint64_t zgpu_plan_slab_bytes(const zgpu_plan_t* p);
+int zgpu_plan_slab_offsets(const zgpu_plan_t* p, uint64_t* slab_off);
+/* the size classes: per class its slab log, block count, bucket bits and parse walkers; returns the class count */
+int zgpu_plan_classes(const zgpu_plan_t* p, uint32_t* block_log, uint32_t* blocks, uint32_t* hash_bits,
+                      uint32_t* walkers, int cap);
+/* encodes every block (on the context's stream; asynchronous): payload at d_slabs + slab_off[b], its size (or
+ * ZGPU_STORED) at d_out_len[b] */
+int zgpu_plan_encode(zgpu_plan_t* p, const uint8_t* d_in, int level, uint8_t* d_slabs, uint32_t* d_out_len);
+/* where the stored blocks are for zgpu_plan_decode: block b's comp_len[b] bytes at offset comp_off[b] of the
+ * compressed buffer, method[b] 0 stored / 1 ZB2 (host arrays, copied) */
+int zgpu_plan_set_stored(zgpu_plan_t* p, const uint64_t* comp_off, const uint32_t* comp_len, const uint8_t* method);
+/* decodes every block: raw bytes to d_out + src[b] (the input layout of zgpu_plan_create), err[b] as
+ * zgpu_decode_blocks */
+int zgpu_plan_decode(zgpu_plan_t* p, const uint8_t* d_comp, uint8_t* d_out, uint32_t* d_err);
+/* times each class's output-invariant launch parameters (bucket bits, parse walkers) on this input and keeps the
+ * fastest (synchronous; writes d_slabs / d_out_len as zgpu_plan_encode does) */
+int zgpu_plan_tune(zgpu_plan_t* p, const uint8_t* d_in, int level, uint8_t* d_slabs, uint32_t* d_out_len);
+/* on: a call's whole launch sequence is captured into a hipGraph the first time and replayed when the same buffers
+ * and level come again */
+int zgpu_plan_use_graphs(zgpu_plan_t* p, int on);
+
 /* ---- stage timing (ZERO_PROFILE's per-stage split, src/gpu_codec.h:32-49): when on, events are recorded between
  * the kernels of every encode/decode call on its stream; zgpu_stage_times reads the last call's (it syncs). */
 int zgpu_set_profile(zgpu_t* g, int on);

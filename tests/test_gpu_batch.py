@@ -1,0 +1,103 @@
+"""The batched form (zgpu_plan_*): blocks of many uneven files planned by size class, through the C ABI. GPU only.
+
+Every payload must be byte-identical to the plain path's for the same block (and so to the oracle's model of the
+reference GPU encoder), whatever the class, the tuned shape or the graph replay; decode must give the files back."""
+import numpy as np
+import pytest
+
+import pyzero
+from oracle_lib import Oracle
+
+pytestmark = pytest.mark.gpu
+O = Oracle()
+
+
+def files(seed):
+    rng = np.random.default_rng(seed)
+    sizes = [0, 1, 3, 4, 17, 100, 1000, 1023, 1024, 1025, 5000, 8192, 8193, 70000, 1 << 18, (1 << 18) + 5, 600000]
+    sizes += [int(x) for x in np.clip(np.exp(np.log(8192) + 2.0 * rng.standard_normal(40)), 64, 1 << 20)]
+    lz = pyzero.gen_lz(seed, sum(sizes) + 1)
+    rnd = pyzero.gen_random(seed + 1, 1 << 16)
+    out, off = [], 0
+    for k, n in enumerate(sizes):
+        if k % 7 == 3:  # some incompressible files, some zeros
+            out.append(np.resize(rnd, n))
+        elif k % 7 == 5:
+            out.append(np.zeros(n, dtype=np.uint8))
+        else:
+            out.append(lz[off: off + n])
+        off += n
+    return out
+
+
+def run_plan(gpu, data, sizes, bl, level, streams=4, tune=False, graphs=0):
+    src, rl, fi = pyzero.file_blocks(sizes, bl)
+    with pyzero.Plan(gpu, src, rl, bl, streams) as plan:
+        d_in = gpu.alloc(max(1, data.size))
+        d_in.upload(data)
+        d_slabs = gpu.alloc(max(1, plan.slab_bytes))
+        d_len = gpu.alloc(max(1, plan.nb) * 4)
+        if tune:
+            plan.tune(d_in.ptr, level, d_slabs.ptr, d_len.ptr)
+        plan.use_graphs(graphs > 0)
+        results = []
+        for _ in range(max(1, graphs)):
+            plan.encode(d_in.ptr, level, d_slabs.ptr, d_len.ptr)
+            gpu.sync()
+            lens = d_len.download(dtype=np.uint32)[: plan.nb].copy()
+            slabs = d_slabs.download()
+            off = plan.slab_offsets()
+            results.append([(0, b"") if lens[b] == pyzero.STORED else
+                            (1, slabs[int(off[b]): int(off[b]) + int(lens[b])].tobytes()) for b in range(plan.nb)])
+            d_slabs.upload(np.zeros(max(1, plan.slab_bytes), dtype=np.uint8))  # (a replay must write it all again)
+        classes = plan.classes()
+        # decode through the plan: payloads back to back
+        blocks = results[-1]
+        comp = [p if m else data[int(s): int(s) + int(r)].tobytes() for (m, p), s, r in zip(blocks, src, rl)]
+        coff = np.cumsum([0] + [len(c) for c in comp])[:-1].astype(np.uint64)
+        plan.set_stored(coff, [len(c) for c in comp], [m for m, _ in blocks])
+        d_comp = gpu.alloc(max(1, sum(len(c) for c in comp)))
+        d_comp.upload(np.frombuffer(b"".join(comp) or b"\0", dtype=np.uint8))
+        d_out = gpu.alloc(max(1, data.size))
+        d_err = gpu.alloc(max(1, plan.nb) * 4)
+        decoded = []
+        for _ in range(max(1, graphs)):
+            d_out.upload(np.zeros(max(1, data.size), dtype=np.uint8))
+            plan.decode(d_comp.ptr, d_out.ptr, d_err.ptr)
+            gpu.sync()
+            assert not d_err.download(dtype=np.uint32)[: plan.nb].any()
+            decoded.append(d_out.download()[: data.size].copy())
+    return results, decoded, classes, (src, rl)
+
+
+@pytest.mark.parametrize("bl,level", [(18, 5), (16, 1), (18, 9)])
+def test_plan_equals_plain_path(gpu, bl, level):
+    fs = files(bl * 10 + level)
+    sizes = [f.size for f in fs]
+    data = np.concatenate(fs)
+    results, decoded, classes, (src, rl) = run_plan(gpu, data, sizes, bl, level)
+    assert len(classes) > 3
+    assert np.array_equal(decoded[0], data)
+    B = 1 << bl
+    k = 0
+    for f in fs:
+        plain = gpu.compress(f, bl, level) if f.size else []
+        for j, want in enumerate(plain):
+            assert results[0][k] == want, (k, f.size, j)
+            if want[0] == 1 and k % 5 == 0:  # and the oracle's model of the reference GPU encoder, on a sample
+                assert want[1] == O.gpu_compress_block(f[j * B:(j + 1) * B].tobytes(), bl, level)[0]
+            k += 1
+    assert k == len(results[0])
+
+
+def test_plan_tuned_and_graph_replays(gpu):
+    fs = files(7)
+    sizes = [f.size for f in fs]
+    data = np.concatenate(fs)
+    base, _, _, _ = run_plan(gpu, data, sizes, 18, 5, streams=1)
+    tuned, dec, classes, _ = run_plan(gpu, data, sizes, 18, 5, streams=3, tune=True, graphs=3)
+    assert len(tuned) == 3 and len(dec) == 3
+    for r in tuned:
+        assert r == base[0]
+    for d in dec:
+        assert np.array_equal(d, data)

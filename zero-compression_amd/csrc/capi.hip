@@ -113,10 +113,23 @@ struct EncLayout {
       lanelen, sinfo, total;
   uint32_t qcap;
 };
-EncLayout enc_layout(uint32_t nb, uint32_t blog) {
+// The launch shape of an encoder call: slabs of 2^blog bytes (every block fits its slab), buckets of the top hbits
+// hash bits, keys of keybits bits, `walkers` parse segments per block. The bytes depend on keybits alone (the
+// archive's block size decides it); blog, hbits and walkers change only the work.
+struct EncShape {
+  uint32_t blog, hbits, keybits, walkers;
+};
+EncShape default_shape(uint32_t blog) { return {blog, hbits_of(blog), hbits_of(blog) + KEY_TAG_BITS, 256}; }
+bool shape_ok(const EncShape& S) {
+  return S.blog >= 10 && S.blog <= 20 && S.hbits >= 4 && S.hbits <= 15 && S.hbits <= S.keybits &&
+         S.keybits <= 27 && S.keybits - S.hbits + S.blog <= 32 && S.walkers >= 1 && S.walkers <= 256 &&
+         (S.walkers & (S.walkers - 1)) == 0 && ((1u << S.blog) / S.walkers) >= 32;
+}
+
+EncLayout enc_layout(uint32_t nb, uint32_t blog, uint32_t hbits) {
   const Geo G = geo_of(blog);
   const size_t N = (size_t)nb << blog;
-  const size_t HB = (size_t)1 << hbits_of(blog);
+  const size_t HB = (size_t)1 << hbits;
   EncLayout L;
   size_t o = 0;
   L.qcap = (uint32_t)((size_t)nb * (G.B / 17u + 1u));
@@ -173,7 +186,9 @@ uint32_t host_mulmod(uint32_t a, uint32_t b) {
 
 extern "C" {
 
-size_t zgpu_encode_scratch_bytes(uint32_t nb, uint32_t block_log) { return enc_layout(nb, block_log).total; }
+size_t zgpu_encode_scratch_bytes(uint32_t nb, uint32_t block_log) {
+  return enc_layout(nb, block_log, hbits_of(block_log)).total;
+}
 size_t zgpu_decode_scratch_bytes(uint32_t nb, uint32_t block_log) { return dec_layout(nb, block_log).total; }
 
 int zgpu_open(int ordinal, zgpu_t** out) {
@@ -325,19 +340,15 @@ int zgpu_stage_times(zgpu_t* g, const char** names, float* ms, int cap) {
 }
 
 // ---------------------------------------------------------------- encode
-int zgpu_encode_blocks(zgpu_t* g, const uint8_t* d_in, const zgpu_block_t* d_blocks, uint32_t nb, uint32_t blog,
-                       int level, uint8_t* d_out, uint32_t* d_out_len, zgpu_stream_t s) {
-  if (blog < 16 || blog > 20 || level < 1 || level > 9) return fail(g, ZGPU_ERR_ARG, "block_log 16..20, level 1..9");
-  if (nb == 0) return ZGPU_OK;
-  hipStream_t st = s ? (hipStream_t)s : g->stream;
-  const EncLayout L = enc_layout(nb, blog);
-  if (int rc = ensure(g, g->enc, L.total)) return rc;
-  uint8_t* base = (uint8_t*)g->enc.p;
-  const Block* blocks = (const Block*)d_blocks;
+// the encoder's launch sequence for nb blocks of one shape, in the scratch at work (enc_layout(nb, S.blog, S.hbits))
+static int encode_run(zgpu_t* g, const uint8_t* d_in, const Block* blocks, uint32_t nb, const EncShape& S, int level,
+                      uint8_t* d_out, uint32_t* d_out_len, uint8_t* base, hipStream_t st, bool prof) {
+  const uint32_t blog = S.blog;
+  const EncLayout L = enc_layout(nb, blog, S.hbits);
   const Geo G = geo_of(blog);
   const uint32_t ncand = level <= 2 ? 2 : level <= 6 ? 4 : 8;
   const uint32_t lencap = level <= 6 ? 16 : 32;
-  const uint32_t HB = 1u << hbits_of(blog);
+  const uint32_t HB = 1u << S.hbits;
   uint32_t* srt = (uint32_t*)(base + L.srt);
   uint32_t* mt = (uint32_t*)(base + L.mtch);
   uint32_t* bstart = (uint32_t*)(base + L.bstart);
@@ -347,51 +358,64 @@ int zgpu_encode_blocks(zgpu_t* g, const uint8_t* d_in, const zgpu_block_t* d_blo
             (uint32_t*)(base + L.offw), (uint32_t*)(base + L.cnt)};
   RansBufs R{base + L.lanes, (uint16_t*)(base + L.freq), (uint32_t*)(base + L.lanelen), (uint32_t*)(base + L.sinfo)};
   const uint32_t posgroups = nb << (blog - 8);
-  stage_begin(g, st);
+  if (prof) stage_begin(g, st);
   k_fill<<<1, 64, 0, st>>>((uint8_t*)q.counts, 0, 16);
-  k_enc_hash<<<nb, 1024, (HB + 1024) * 4, st>>>(blocks, d_in, blog, srt, bstart);
-  stage(g, st, "hash");
-  k_enc_sort_s<<<(unsigned)(((size_t)nb * HB + 255) / 256), 256, 0, st>>>(blocks, nb, blog, srt, bstart, q);
-  k_enc_sort_m<<<std::min<uint32_t>(2048, L.qcap), 256, 0, st>>>(blog, srt, bstart, q);
-  k_enc_sort_l<<<std::min<uint32_t>(1024, L.qcap), 1024, 0, st>>>(blog, srt, mt, bstart, q);
-  stage(g, st, "sort");
-  k_enc_match<<<posgroups, 256, 0, st>>>(blocks, d_in, blog, ncand, lencap, srt, bstart, mt);
-  stage(g, st, "match");
+  k_enc_hash<<<nb, 1024, (HB + 1024) * 4, st>>>(blocks, d_in, blog, S.hbits, S.keybits, srt, bstart);
+  if (prof) stage(g, st, "hash");
+  k_enc_sort_s<<<(unsigned)(((size_t)nb * HB + 255) / 256), 256, 0, st>>>(blocks, nb, blog, S.hbits, srt, bstart, q);
+  k_enc_sort_m<<<std::min<uint32_t>(2048, L.qcap), 256, 0, st>>>(blog, S.hbits, srt, bstart, q);
+  k_enc_sort_l<<<std::min<uint32_t>(1024, L.qcap), 1024, 0, st>>>(blog, S.hbits, srt, mt, bstart, q);
+  if (prof) stage(g, st, "sort");
+  k_enc_match<<<posgroups, 256, 0, st>>>(blocks, d_in, blog, S.hbits, ncand, lencap, srt, bstart, mt);
+  if (prof) stage(g, st, "match");
   k_enc_lazy<<<posgroups, 256, 0, st>>>(blocks, blog, mt, srt);
-  stage(g, st, "lazy");
-  const size_t parse_lds = ((size_t)G.B / 32 + 256 + G.B / 256) * 4;
-  k_enc_parse<<<nb, 256, parse_lds, st>>>(blocks, d_in, blog, lencap, srt, (uint32_t*)(base + L.mark));
-  stage(g, st, "parse");
+  if (prof) stage(g, st, "lazy");
+  const size_t parse_lds = ((size_t)G.B / 32 + 256 + G.B / S.walkers) * 4;
+  k_enc_parse<<<nb, 256, parse_lds, st>>>(blocks, d_in, blog, S.walkers, lencap, srt, (uint32_t*)(base + L.mark));
+  if (prof) stage(g, st, "parse");
   k_enc_tokens<<<nb, 64, 0, st>>>(blocks, d_in, blog, srt, (uint32_t*)(base + L.mark), T);
-  stage(g, st, "tokens");
+  if (prof) stage(g, st, "tokens");
   k_enc_rans<<<nb * 5, 64, 0, st>>>(blog, T, R);
-  stage(g, st, "rans");
+  if (prof) stage(g, st, "rans");
   k_enc_pack<<<nb, 256, 0, st>>>(blocks, blog, T, R, d_out, d_out_len);
-  stage(g, st, "pack");
+  if (prof) stage(g, st, "pack");
   HIPCHK(g, hipGetLastError());
   return ZGPU_OK;
 }
 
+int zgpu_encode_blocks(zgpu_t* g, const uint8_t* d_in, const zgpu_block_t* d_blocks, uint32_t nb, uint32_t blog,
+                       int level, uint8_t* d_out, uint32_t* d_out_len, zgpu_stream_t s) {
+  if (blog < 16 || blog > 20 || level < 1 || level > 9) return fail(g, ZGPU_ERR_ARG, "block_log 16..20, level 1..9");
+  if (nb == 0) return ZGPU_OK;
+  hipStream_t st = s ? (hipStream_t)s : g->stream;
+  const EncShape S = default_shape(blog);
+  if (int rc = ensure(g, g->enc, enc_layout(nb, blog, S.hbits).total)) return rc;
+  return encode_run(g, d_in, (const Block*)d_blocks, nb, S, level, d_out, d_out_len, (uint8_t*)g->enc.p, st, true);
+}
+
 // ---------------------------------------------------------------- decode
+static int decode_run(zgpu_t* g, const uint8_t* d_comp, const Block* blocks, uint32_t nb, uint32_t blog,
+                      uint8_t* d_out, uint32_t* d_err, uint8_t* base, hipStream_t st, bool prof) {
+  const DecLayout L = dec_layout(nb, blog);
+  if (prof) stage_begin(g, st);
+  k_fill<<<(unsigned)std::min<size_t>((nb * 4 + 255) / 256, 1024), 256, 0, st>>>((uint8_t*)d_err, 0, (size_t)nb * 4);
+  k_dec_streams<<<nb * 5, 64, 0, st>>>(blocks, d_comp, blog, base + L.lit, base + L.tok, base + L.ext, base + L.sym,
+                                       base + L.grp, d_err);
+  if (prof) stage(g, st, "streams");
+  k_dec_lz<<<nb, 64, 0, st>>>(blocks, d_comp, blog, base + L.lit, base + L.tok, base + L.ext, base + L.sym,
+                              base + L.grp, d_out, d_err);
+  if (prof) stage(g, st, "lz");
+  HIPCHK(g, hipGetLastError());
+  return ZGPU_OK;
+}
+
 int zgpu_decode_blocks(zgpu_t* g, const uint8_t* d_comp, const zgpu_block_t* d_blocks, uint32_t nb, uint32_t blog,
                        uint8_t* d_out, uint32_t* d_err, zgpu_stream_t s) {
   if (blog < 16 || blog > 20) return fail(g, ZGPU_ERR_ARG, "block_log 16..20");
   if (nb == 0) return ZGPU_OK;
   hipStream_t st = s ? (hipStream_t)s : g->stream;
-  const DecLayout L = dec_layout(nb, blog);
-  if (int rc = ensure(g, g->dec, L.total)) return rc;
-  uint8_t* base = (uint8_t*)g->dec.p;
-  const Block* blocks = (const Block*)d_blocks;
-  stage_begin(g, st);
-  k_fill<<<(unsigned)std::min<size_t>((nb * 4 + 255) / 256, 1024), 256, 0, st>>>((uint8_t*)d_err, 0, (size_t)nb * 4);
-  k_dec_streams<<<nb * 5, 64, 0, st>>>(blocks, d_comp, blog, base + L.lit, base + L.tok, base + L.ext, base + L.sym,
-                                       base + L.grp, d_err);
-  stage(g, st, "streams");
-  k_dec_lz<<<nb, 64, 0, st>>>(blocks, d_comp, blog, base + L.lit, base + L.tok, base + L.ext, base + L.sym,
-                              base + L.grp, d_out, d_err);
-  stage(g, st, "lz");
-  HIPCHK(g, hipGetLastError());
-  return ZGPU_OK;
+  if (int rc = ensure(g, g->dec, dec_layout(nb, blog).total)) return rc;
+  return decode_run(g, d_comp, (const Block*)d_blocks, nb, blog, d_out, d_err, (uint8_t*)g->dec.p, st, true);
 }
 
 int zgpu_crc32_blocks(zgpu_t* g, const uint8_t* d_data, const zgpu_block_t* d_blocks, uint32_t nb, int use_dst,
@@ -517,3 +541,5 @@ int zgpu_crc32(zgpu_t* g, const uint8_t* data, uint64_t n, uint32_t blog, uint32
 }
 
 }  // extern "C"
+
+#include "plan.hip"

@@ -51,14 +51,16 @@ __device__ inline uint32_t prefix_len(const uint8_t* inb, uint32_t a, uint32_t b
 __device__ inline uint32_t bitlen(uint32_t d) { return 31u - (uint32_t)__builtin_clz(d); }
 
 // ---------------------------------------------------------------- 1. hash, count, scatter (one workgroup per block)
-// srt[b*B + slot] = (12 tag bits of the key) << blog | position, grouped by bucket (the top hbits of the hash);
-// bstart[b*(HB+1) + h] = the first slot of bucket h.
+// The key of a position is the top `keybits` bits of its hash; buckets are its top `hbits` bits. srt[b*B + slot] =
+// (the key's other keybits - hbits bits) << blog | position, grouped by bucket; bstart[b*(HB+1) + h] = the first slot
+// of bucket h. Once each bucket is sorted, positions are in (key, position) order whatever hbits is, so hbits (and
+// the slab size 2^blog, any size that holds the block) changes the work, not the candidates.
 __global__ __launch_bounds__(1024) void k_enc_hash(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
-                                                   uint32_t blog, uint32_t* __restrict__ srt,
-                                                   uint32_t* __restrict__ bstart) {
+                                                   uint32_t blog, uint32_t hbits, uint32_t keybits,
+                                                   uint32_t* __restrict__ srt, uint32_t* __restrict__ bstart) {
   extern __shared__ uint32_t cnt[];  // HB counters, then HB/1024 partial sums
   const uint32_t b = blockIdx.x, tid = threadIdx.x;
-  const uint32_t HBITS = hbits_of(blog), HB = 1u << HBITS;
+  const uint32_t HBITS = hbits, HB = 1u << HBITS;
   const Block blk = blocks[b];
   const uint8_t* inb = in + blk.src;
   const uint32_t n = blk.raw_len;
@@ -99,11 +101,11 @@ __global__ __launch_bounds__(1024) void k_enc_hash(const Block* __restrict__ blo
   if (tid == 0) bs[HB] = count;
   __syncthreads();
   uint32_t* sb = srt + ((size_t)b << blog);
-  const uint32_t keybits = HBITS + KEY_TAG_BITS;
+  const uint32_t tagmask = (1u << (keybits - HBITS)) - 1u;
   for (uint32_t p = tid; p < count; p += 1024) {
     const uint32_t h = hash5(inb, n, p);
     const uint32_t slot = atomicAdd(&cnt[h >> (32 - HBITS)], 1u);
-    sb[slot] = ((h >> (32 - keybits)) & ((1u << KEY_TAG_BITS) - 1u)) << blog | p;
+    sb[slot] = ((h >> (32 - keybits)) & tagmask) << blog | p;
   }
 }
 
@@ -119,9 +121,9 @@ struct SortQueues {
 // wavefront sorts them one after another, one entry per lane, a bitonic network over lane shuffles. Larger buckets
 // are queued for k_enc_sort_m (<= 4096) or k_enc_sort_l.
 __global__ __launch_bounds__(256) void k_enc_sort_s(const Block* __restrict__ blocks, uint32_t nb, uint32_t blog,
-                                                    uint32_t* __restrict__ srt, const uint32_t* __restrict__ bstart,
-                                                    SortQueues q) {
-  const uint32_t HB = 1u << hbits_of(blog);
+                                                    uint32_t hbits, uint32_t* __restrict__ srt,
+                                                    const uint32_t* __restrict__ bstart, SortQueues q) {
+  const uint32_t HB = 1u << hbits;
   const uint64_t gid = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   const uint32_t b = (uint32_t)(gid / HB), h = (uint32_t)(gid % HB);
   const bool valid = b < nb;
@@ -179,10 +181,10 @@ __global__ __launch_bounds__(256) void k_enc_sort_s(const Block* __restrict__ bl
 }
 
 // queued buckets of 65..4096 entries: one workgroup per bucket, bitonic sort in LDS (padded to a power of two)
-__global__ __launch_bounds__(256) void k_enc_sort_m(uint32_t blog, uint32_t* __restrict__ srt,
+__global__ __launch_bounds__(256) void k_enc_sort_m(uint32_t blog, uint32_t hbits, uint32_t* __restrict__ srt,
                                                     const uint32_t* __restrict__ bstart, SortQueues q) {
   __shared__ uint32_t a[4096];
-  const uint32_t HB = 1u << hbits_of(blog);
+  const uint32_t HB = 1u << hbits;
   const uint32_t total = min(q.counts[0], q.cap);
   for (uint32_t w = blockIdx.x; w < total; w += gridDim.x) {
     const uint32_t b = q.mid[2 * w], h = q.mid[2 * w + 1];
@@ -212,12 +214,12 @@ __global__ __launch_bounds__(256) void k_enc_sort_m(uint32_t blog, uint32_t* __r
 
 // queued buckets of more than 4096 entries: one workgroup of 1024 per bucket, a stable LSD radix sort with 8-bit
 // digits through a temporary of the same size (tmp, indexed like srt), in-wavefront ranks from ballots.
-__global__ __launch_bounds__(1024) void k_enc_sort_l(uint32_t blog, uint32_t* __restrict__ srt,
+__global__ __launch_bounds__(1024) void k_enc_sort_l(uint32_t blog, uint32_t hbits, uint32_t* __restrict__ srt,
                                                      uint32_t* __restrict__ tmp, const uint32_t* __restrict__ bstart,
                                                      SortQueues q) {
   __shared__ uint32_t base[256], tile_tot[256];
   __shared__ uint32_t wcnt[16][256];
-  const uint32_t HB = 1u << hbits_of(blog);
+  const uint32_t HB = 1u << hbits;
   const uint32_t tid = threadIdx.x, wv = tid >> 6;
   const uint32_t total = min(q.counts[1], q.cap);
   for (uint32_t w = blockIdx.x; w < total; w += gridDim.x) {
@@ -276,7 +278,7 @@ __global__ __launch_bounds__(1024) void k_enc_sort_l(uint32_t blog, uint32_t* __
 
 // ---------------------------------------------------------------- 3. match finder (one thread per sorted slot)
 __global__ __launch_bounds__(256) void k_enc_match(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
-                                                   uint32_t blog, uint32_t ncand, uint32_t lencap,
+                                                   uint32_t blog, uint32_t hbits, uint32_t ncand, uint32_t lencap,
                                                    const uint32_t* __restrict__ srt, const uint32_t* __restrict__ bstart,
                                                    uint32_t* __restrict__ match) {
   const uint32_t B = 1u << blog;
@@ -290,7 +292,7 @@ __global__ __launch_bounds__(256) void k_enc_match(const Block* __restrict__ blo
   if (i >= count) { mb[i] = 0; return; }  // (the last 3 positions are never in srt: exactly these slots)
   const uint8_t* inb = in + blk.src;
   const uint32_t* sb = srt + ((size_t)b << blog);
-  const uint32_t HBITS = hbits_of(blog), HB = 1u << HBITS;
+  const uint32_t HBITS = hbits, HB = 1u << HBITS;
   const uint32_t e = sb[i];
   const uint32_t p = e & (B - 1u), tag = e >> blog;
   if (p >= count) return;  // (never with a correct sort; keeps every read inside the block)
@@ -360,15 +362,16 @@ __device__ inline uint32_t nextpos(uint32_t* m, const uint8_t* inb, uint32_t n, 
   return p + (l ? l : 1u);
 }
 
-// Each of the 256 threads walks its segment from the segment's start, marking token starts in an LDS bitmap.
+// Each of the first `walkers` threads (a power of two, 32 <= B / walkers) walks its segment of B / walkers positions
+// from the segment's start, marking token starts in an LDS bitmap.
 // Thread 0 then follows the true chain: where it meets a walk the two coincide from there on (the chain is a pure
 // function of the position), so only the walk's prefix is wrong; where it never meets one, the walk is wrong whole.
 // mark[b] receives the bitmap of the chain (B bits).
 __global__ __launch_bounds__(256) void k_enc_parse(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
-                                                   uint32_t blog, uint32_t lencap, uint32_t* __restrict__ lazy,
-                                                   uint32_t* __restrict__ mark) {
+                                                   uint32_t blog, uint32_t walkers, uint32_t lencap,
+                                                   uint32_t* __restrict__ lazy, uint32_t* __restrict__ mark) {
   extern __shared__ uint32_t sm[];  // bitmap (B/32 words), then ex[256], then the join's path (SEG entries)
-  const uint32_t B = 1u << blog, SEG = B / 256u;
+  const uint32_t B = 1u << blog, SEG = B / walkers;
   const uint32_t b = blockIdx.x, k = threadIdx.x;
   uint32_t* bm = sm;
   uint32_t* ex = sm + B / 32u;
@@ -379,7 +382,7 @@ __global__ __launch_bounds__(256) void k_enc_parse(const Block* __restrict__ blo
   uint32_t* m = lazy + ((size_t)b << blog);
   for (uint32_t i = k; i < B / 32u; i += 256) bm[i] = 0;
   __syncthreads();
-  {
+  if (k < walkers) {
     const uint32_t s = k * SEG, send = min(s + SEG, n);
     uint32_t p = s;
     while (p < send) {
@@ -391,7 +394,7 @@ __global__ __launch_bounds__(256) void k_enc_parse(const Block* __restrict__ blo
   __syncthreads();
   if (k == 0) {
     uint32_t e = 0;
-    for (uint32_t q = 0; q < 256; ++q) {
+    for (uint32_t q = 0; q < walkers; ++q) {
       const uint32_t s = q * SEG;
       if (s >= n) break;
       const uint32_t send = min(s + SEG, n);

@@ -33,6 +33,8 @@ EXPORTS = (
     "zgpu_encode_scratch_bytes", "zgpu_decode_scratch_bytes",
     "zgpu_compress", "zgpu_decompress", "zgpu_crc32",
     "zgpu_set_profile", "zgpu_stage_times",
+    "zgpu_plan_create", "zgpu_plan_destroy", "zgpu_plan_slab_bytes", "zgpu_plan_slab_offsets", "zgpu_plan_classes",
+    "zgpu_plan_encode", "zgpu_plan_set_stored", "zgpu_plan_decode", "zgpu_plan_tune", "zgpu_plan_use_graphs",
 )
 
 BLOCK_DTYPE = np.dtype([("src", "<u8"), ("dst", "<u8"), ("raw_len", "<u4"), ("comp_len", "<u4"),
@@ -86,6 +88,12 @@ def load_library(path=LIB_PATH):
         "zgpu_upload": (i32, [vp, vp, vp, ctypes.c_size_t]), "zgpu_download": (i32, [vp, vp, vp, ctypes.c_size_t]),
         "zgpu_fill": (i32, [vp, vp, ctypes.c_uint8, ctypes.c_size_t, vp]),
         "zgpu_copy": (i32, [vp, vp, vp, ctypes.c_size_t, vp]),
+        "zgpu_plan_create": (i32, [vp, u32, vp, vp, u32, i32, ctypes.POINTER(vp)]),
+        "zgpu_plan_destroy": (None, [vp]), "zgpu_plan_slab_bytes": (ctypes.c_uint64, [vp]),
+        "zgpu_plan_slab_offsets": (i32, [vp, vp]), "zgpu_plan_classes": (i32, [vp, vp, vp, vp, vp, i32]),
+        "zgpu_plan_encode": (i32, [vp, vp, i32, vp, vp]), "zgpu_plan_set_stored": (i32, [vp, vp, vp, vp]),
+        "zgpu_plan_decode": (i32, [vp, vp, vp, vp]), "zgpu_plan_tune": (i32, [vp, vp, i32, vp, vp]),
+        "zgpu_plan_use_graphs": (i32, [vp, i32]),
         "zgpu_stream_create": (i32, [vp, ctypes.POINTER(vp)]), "zgpu_stream_destroy": (i32, [vp, vp]),
         "zgpu_stream_sync": (i32, [vp, vp]),
         "zgpu_event_create": (i32, [vp, ctypes.POINTER(vp)]), "zgpu_event_destroy": (i32, [vp, vp]),
@@ -273,6 +281,80 @@ class Gpu:
         if k < 0:
             self._check(k)
         return [(names[i].decode(), ms[i]) for i in range(k)]
+
+
+class Plan:
+    """The batched form (zgpu_plan_*): the blocks of many files of uneven lengths, planned together. src/raw_len
+    give every block's place in the input buffer; block_log is the archive's block size."""
+
+    def __init__(self, gpu, src, raw_len, block_log, streams=4):
+        self.gpu, self.lib = gpu, gpu.lib
+        self.src = np.ascontiguousarray(src, dtype=np.uint64)
+        self.raw_len = np.ascontiguousarray(raw_len, dtype=np.uint32)
+        self.nb = int(self.raw_len.size)
+        self.block_log = block_log
+        h = vp()
+        gpu._check(self.lib.zgpu_plan_create(gpu.h, self.nb, _ptr(self.src), _ptr(self.raw_len), block_log, streams,
+                                             ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            self.lib.zgpu_plan_destroy(self.h)
+        self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def slab_bytes(self):
+        return int(self.lib.zgpu_plan_slab_bytes(self.h))
+
+    def slab_offsets(self):
+        off = np.empty(max(1, self.nb), dtype=np.uint64)
+        self.gpu._check(self.lib.zgpu_plan_slab_offsets(self.h, _ptr(off)))
+        return off[: self.nb]
+
+    def classes(self):
+        cap = 16
+        a = [np.zeros(cap, dtype=np.uint32) for _ in range(4)]
+        n = self.lib.zgpu_plan_classes(self.h, *[_ptr(x) for x in a], cap)
+        return [dict(block_log=int(a[0][i]), blocks=int(a[1][i]), hash_bits=int(a[2][i]), walkers=int(a[3][i]))
+                for i in range(min(n, cap))]
+
+    def encode(self, d_in, level, d_slabs, d_out_len):
+        self.gpu._check(self.lib.zgpu_plan_encode(self.h, d_in, level, d_slabs, d_out_len))
+
+    def set_stored(self, comp_off, comp_len, method):
+        self._st = [np.ascontiguousarray(comp_off, dtype=np.uint64), np.ascontiguousarray(comp_len, dtype=np.uint32),
+                    np.ascontiguousarray(method, dtype=np.uint8)]
+        self.gpu._check(self.lib.zgpu_plan_set_stored(self.h, *[_ptr(x) for x in self._st]))
+
+    def decode(self, d_comp, d_out, d_err):
+        self.gpu._check(self.lib.zgpu_plan_decode(self.h, d_comp, d_out, d_err))
+
+    def tune(self, d_in, level, d_slabs, d_out_len):
+        self.gpu._check(self.lib.zgpu_plan_tune(self.h, d_in, level, d_slabs, d_out_len))
+
+    def use_graphs(self, on=True):
+        self.gpu._check(self.lib.zgpu_plan_use_graphs(self.h, 1 if on else 0))
+
+
+def file_blocks(sizes, block_log):
+    """each file cut into blocks of 2^block_log (the last one shorter), files back to back: (src, raw_len, file)"""
+    B = 1 << block_log
+    src, rl, fi = [], [], []
+    off = 0
+    for k, n in enumerate(sizes):
+        for o in range(0, int(n), B):
+            src.append(off + o)
+            rl.append(min(B, int(n) - o))
+            fi.append(k)
+        off += int(n)
+    return (np.array(src, dtype=np.uint64), np.array(rl, dtype=np.uint32), np.array(fi, dtype=np.uint32))
 
 
 def block_table(raw_lens, comp_lens=None, methods=None, src=None, dst=None):
