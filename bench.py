@@ -10,6 +10,10 @@ launch duration against the 8 TB/s HBM peak. ``cpu_baseline`` is the reference's
 from the reference's headers) on a bounded sample of the same input, on this machine's cores.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--gib G]
+
+``--workload uneven`` measures the batched form instead: 200,000 files with lognormal sizes (median 8 KiB, sigma 2,
+clipped to 64 B .. 64 MiB) cut from the seed-1 LZ stream, level 5, 2^18 blocks, through one size-class plan
+(autotuned, replayed as a hipGraph), against the plain path called once per file on a sample of the same files.
 """
 import argparse
 import json
@@ -73,6 +77,132 @@ def traffic_of(kernel, n, level, bl):
     return None if k is None else k["hbm_bytes_per_launch"]
 
 
+def bench_uneven(a, gpu):
+    """The batched uneven-file form against per-file calls of the plain path (one JSON line)."""
+    import pyzero
+
+    bl, level = a.block_log, a.level
+    B = 1 << bl
+    sizes = pyzero.gen_lognormal_sizes(4, 8192.0, 2.0, 64, 1 << 26, a.files).astype(np.int64)
+    total = int(sizes.sum())
+    t0 = time.time()
+    data = pyzero.gen_lz(1, total)
+    log(f"uneven: {a.files} files, {total / 2**30:.2f} GiB in total (generated in {time.time() - t0:.1f} s)")
+    src, rl, fi = pyzero.file_blocks(sizes, bl)
+    nb = int(rl.size)
+    d_in = gpu.alloc(total)
+    d_in.upload(data)
+    ev0, ev1 = gpu.event(), gpu.event()
+
+    def timed(fn, steps):
+        gpu.sync()
+        gpu.record(ev0)
+        for _ in range(steps):
+            fn()
+        gpu.record(ev1)
+        gpu.sync()
+        return gpu.elapsed_ms(ev0, ev1) / steps
+
+    plan = pyzero.Plan(gpu, src, rl, bl, a.streams)
+    d_slabs = gpu.alloc(plan.slab_bytes)
+    d_len = gpu.alloc(nb * 4)
+    t0 = time.time()
+    plan.tune(d_in.ptr, level, d_slabs.ptr, d_len.ptr)
+    log(f"uneven: tuned in {time.time() - t0:.1f} s; classes {plan.classes()}")
+    plan.use_graphs(True)
+    for _ in range(max(1, a.warmup)):
+        plan.encode(d_in.ptr, level, d_slabs.ptr, d_len.ptr)
+    enc_ms = timed(lambda: plan.encode(d_in.ptr, level, d_slabs.ptr, d_len.ptr), a.steps)
+    lens = d_len.download(dtype=np.uint32).copy()
+    stored = lens == pyzero.STORED
+    clen = np.where(stored, rl, lens).astype(np.uint32)
+    slabs = d_slabs.download()
+    off = plan.slab_offsets()
+    comp = np.concatenate([data[int(s): int(s) + int(r)] if st else slabs[int(o): int(o) + int(c)]
+                           for s, r, st, o, c in zip(src, rl, stored, off, clen)])
+    del slabs
+    coff = np.concatenate([[0], np.cumsum(clen.astype(np.uint64))[:-1]]).astype(np.uint64)
+    plan.set_stored(coff, clen, np.where(stored, 0, 1))
+    d_comp = gpu.alloc(comp.size)
+    d_comp.upload(comp)
+    d_out = gpu.alloc(total)
+    d_err = gpu.alloc(nb * 4)
+    for _ in range(max(1, a.warmup)):
+        plan.decode(d_comp.ptr, d_out.ptr, d_err.ptr)
+    dec_ms = timed(lambda: plan.decode(d_comp.ptr, d_out.ptr, d_err.ptr), a.steps)
+    ok = not d_err.download(dtype=np.uint32).any() and np.array_equal(d_out.download(), data)
+    log(f"uneven batched: encode {total / enc_ms / 1e6:.1f} GB/s, decode {total / dec_ms / 1e6:.1f} GB/s, ratio "
+        f"{comp.size / total:.4f}, round trip {'identical' if ok else 'MISMATCH'}")
+
+    # the plain path, one call per file, on the first files (device-resident, same buffers)
+    nfs = min(a.files, a.sample_files)
+    nbs = int(np.searchsorted(fi, nfs))
+    sample_bytes = int(rl[:nbs].sum())
+    tab = pyzero.block_table(rl[:nbs], src=src[:nbs], dst=off[:nbs])
+    d_tab = gpu.alloc(max(1, tab.nbytes))
+    d_tab.upload(tab)
+    dtab = pyzero.block_table(rl[:nbs], comp_lens=clen[:nbs], methods=np.where(stored[:nbs], 0, 1),
+                              src=coff[:nbs], dst=src[:nbs])
+    d_dtab = gpu.alloc(max(1, dtab.nbytes))
+    d_dtab.upload(dtab)
+    firsts = np.searchsorted(fi[:nbs], np.arange(nfs + 1))
+    d_pslab = gpu.alloc(max(1, nbs) * B)
+    d_plen = gpu.alloc(max(1, nbs) * 4)
+
+    def perfile_encode():
+        for k in range(nfs):
+            b0, b1 = int(firsts[k]), int(firsts[k + 1])
+            if b1 > b0:
+                gpu.encode_blocks(d_in.ptr, d_tab.addr(b0 * 32), b1 - b0, bl, level, d_pslab.addr(b0 * B),
+                                  d_plen.addr(b0 * 4))
+
+    def perfile_decode():
+        for k in range(nfs):
+            b0, b1 = int(firsts[k]), int(firsts[k + 1])
+            if b1 > b0:
+                gpu.decode_blocks(d_comp.ptr, d_dtab.addr(b0 * 32), b1 - b0, bl, d_out.ptr, d_err.addr(b0 * 4))
+
+    perfile_encode()
+    pf_enc_ms = timed(perfile_encode, 1)
+    plen = d_plen.download(nbs * 4, dtype=np.uint32)
+    same = bool(np.array_equal(plen, lens[:nbs]))
+    perfile_decode()
+    pf_dec_ms = timed(perfile_decode, 1)
+    # the batched form on the same sample
+    with pyzero.Plan(gpu, src[:nbs], rl[:nbs], bl, a.streams) as sp:
+        d_s2 = gpu.alloc(max(1, sp.slab_bytes))
+        sp.tune(d_in.ptr, level, d_s2.ptr, d_len.ptr)
+        sp.use_graphs(True)
+        sp.encode(d_in.ptr, level, d_s2.ptr, d_len.ptr)
+        sb_enc_ms = timed(lambda: sp.encode(d_in.ptr, level, d_s2.ptr, d_len.ptr), a.steps)
+        sp.set_stored(coff[:nbs], clen[:nbs], np.where(stored[:nbs], 0, 1))
+        sp.decode(d_comp.ptr, d_out.ptr, d_err.ptr)
+        sb_dec_ms = timed(lambda: sp.decode(d_comp.ptr, d_out.ptr, d_err.ptr), a.steps)
+    log(f"sample of {nfs} files ({sample_bytes / 2**20:.1f} MiB): per-file encode {sample_bytes / pf_enc_ms / 1e6:.2f} "
+        f"GB/s, decode {sample_bytes / pf_dec_ms / 1e6:.2f} GB/s; batched encode {sample_bytes / sb_enc_ms / 1e6:.2f} "
+        f"GB/s, decode {sample_bytes / sb_dec_ms / 1e6:.2f} GB/s; per-file payloads identical: {same}")
+    line = {
+        "metric": "batched uneven-file compress/decompress GB/s vs the plain per-file form",
+        "value": round(total / (dec_ms * 1e-3) / 1e9, 3), "unit": "GB/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(dec_ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u8", "data": "synthetic (seeded generators; no dataset)",
+        "config": {"workload": f"{a.files} files, lognormal sizes (median 8 KiB, sigma 2, 64 B..64 MiB), seed-1 LZ "
+                               f"bytes, level {level}, 2^{bl} blocks, one plan; step = decode of every file",
+                   "files": a.files, "input_bytes": total, "blocks": nb, "streams": a.streams},
+        "batched": {"compress_GBps": round(total / (enc_ms * 1e-3) / 1e9, 3),
+                    "decompress_GBps": round(total / (dec_ms * 1e-3) / 1e9, 3), "ratio": round(comp.size / total, 5),
+                    "roundtrip_identical": ok, "classes": plan.classes()},
+        "per_file_sample": {"files": nfs, "bytes": sample_bytes,
+                            "per_file_compress_GBps": round(sample_bytes / (pf_enc_ms * 1e-3) / 1e9, 3),
+                            "per_file_decompress_GBps": round(sample_bytes / (pf_dec_ms * 1e-3) / 1e9, 3),
+                            "batched_compress_GBps": round(sample_bytes / (sb_enc_ms * 1e-3) / 1e9, 3),
+                            "batched_decompress_GBps": round(sample_bytes / (sb_dec_ms * 1e-3) / 1e9, 3),
+                            "payloads_identical": same},
+    }
+    plan.close()
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +212,10 @@ def main():
     ap.add_argument("--level", type=int, default=5)
     ap.add_argument("--block-log", type=int, default=18)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["plain", "uneven"], default="plain")
+    ap.add_argument("--files", type=int, default=200000, help="uneven workload: number of files")
+    ap.add_argument("--sample-files", type=int, default=2000, help="uneven workload: files timed per-file")
+    ap.add_argument("--streams", type=int, default=4, help="uneven workload: streams of the plan")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (blocks would shard across GPUs with one "
@@ -96,6 +230,10 @@ def main():
     gpu = pyzero.Gpu(0)
     props = gpu.props()
     log(f"device {props['name']} {props['arch']} {props['cu_count']} CUs")
+    if a.workload == "uneven":
+        bench_uneven(a, gpu)
+        gpu.close()
+        return
     t0 = time.time()
     data = pyzero.gen_lz(1, n)
     rnd = pyzero.gen_random(2, 1 << 30)
