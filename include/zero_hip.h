@@ -39,15 +39,16 @@ typedef struct zgpu zgpu_t;
 typedef void* zgpu_stream_t; /* a hipStream_t; NULL is the context's own stream */
 
 /* One entry per block of a launch (32 bytes, device memory for the zgpu_*_blocks calls). Encoding reads raw_len bytes
- * at in + src. Decoding reads comp_len bytes at comp + src (method 0 stored, 1 ZB2) and writes raw_len bytes at
- * out + dst. Mirrors zero::BlockIn (src/archive.h:50-55) plus where the bytes live. */
+ * at in + src, at `level` (1..9) when it is not 0, else at the call's level. Decoding reads comp_len bytes at
+ * comp + src (method 0 stored, 1 ZB2) and writes raw_len bytes at out + dst. Mirrors zero::BlockIn
+ * (src/archive.h:50-55) plus where the bytes live. */
 typedef struct {
   uint64_t src;
   uint64_t dst;
   uint32_t raw_len;
   uint32_t comp_len;
   uint32_t method;
-  uint32_t pad;
+  uint32_t level;
 } zgpu_block_t;
 
 typedef struct {
@@ -155,6 +156,36 @@ int zgpu_plan_tune(zgpu_plan_t* p, const uint8_t* d_in, int level, uint8_t* d_sl
 /* on: a call's whole launch sequence is captured into a hipGraph the first time and replayed when the same buffers
  * and level come again */
 int zgpu_plan_use_graphs(zgpu_plan_t* p, int on);
+
+/* ---- the serving loop (no reference counterpart: the reference has no server). A request loop admits compress and
+ * decompress requests of mixed sizes and levels into batches by HBM budget while earlier batches run (continuous
+ * batching over `slots` batch slots, each with its own stream and preallocated device arenas), and reports latency
+ * from arrival to the result in host memory. zgpu_serve_bench drives it with a synthetic open-loop load: Poisson
+ * arrivals at offered_GBps of raw bytes for `seconds`, lognormal sizes, compress_share compress requests (level
+ * uniform over 1 / 5 / 9), inputs cut from the two pools in random_share : 1 - random_share (decompress inputs are
+ * the pools' blocks encoded at level 5 before the clock starts). After the run every response is round-tripped,
+ * and alone_checks requests are served again one per batch and compared with their results in the mix. */
+typedef struct {
+  double offered_GBps, seconds, compress_share, random_share;
+  double size_median, size_sigma;
+  uint64_t size_max;
+  uint64_t hbm_budget;
+  uint32_t block_log;
+  int slots;
+  uint64_t seed;
+  uint32_t alone_checks;
+} zgpu_serve_cfg_t;
+typedef struct {
+  uint64_t requests, compress_requests, decompress_requests, bytes;
+  double p50_ms, p99_ms, max_ms, mean_ms;
+  double wall_s, sustained_GBps;
+  uint64_t batches;
+  double mean_batch_requests;
+  uint64_t hbm_bytes, slot_input_bytes;
+  uint64_t mismatches, alone_checked, alone_mismatches;
+} zgpu_serve_report_t;
+int zgpu_serve_bench(zgpu_t* g, const zgpu_serve_cfg_t* cfg, const uint8_t* lz_pool, uint64_t lz_len,
+                     const uint8_t* random_pool, uint64_t random_len, zgpu_serve_report_t* rep);
 
 /* ---- stage timing (ZERO_PROFILE's per-stage split, src/gpu_codec.h:32-49): when on, events are recorded between
  * the kernels of every encode/decode call on its stream; zgpu_stage_times reads the last call's (it syncs). */

@@ -101,3 +101,17 @@ def test_plan_tuned_and_graph_replays(gpu):
         assert r == base[0]
     for d in dec:
         assert np.array_equal(d, data)
+
+
+def test_serving_loop_round_trips_and_mix_independence(gpu):
+    """a short open-loop run: every request completes, every response round-trips, and requests served again one per
+    batch give the same bytes as in the mix"""
+    lz = pyzero.gen_lz(1, 64 << 20)
+    rnd = pyzero.gen_random(2, 8 << 20)
+    rep = gpu.serve_bench(lz, rnd, offered_GBps=4.0, seconds=0.25, size_median=64 * 1024, size_sigma=1.5,
+                          size_max=8 << 20, hbm_budget=8 << 30, slots=2, seed=11, alone_checks=24)
+    assert rep["requests"] > 100
+    assert rep["compress_requests"] > 0 and rep["decompress_requests"] > 0
+    assert rep["mismatches"] == 0
+    assert rep["alone_checked"] == 24 and rep["alone_mismatches"] == 0
+    assert rep["batches"] >= 2 and rep["p50_ms"] <= rep["p99_ms"] <= rep["max_ms"]

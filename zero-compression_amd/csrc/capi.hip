@@ -441,7 +441,7 @@ int zgpu_compress(zgpu_t* g, const uint8_t* in, uint64_t n, uint32_t blog, int l
     tab[b].raw_len = (uint32_t)std::min<uint64_t>(B, n - (uint64_t)b * B);
     tab[b].comp_len = 0;
     tab[b].method = 0;
-    tab[b].pad = 0;
+    tab[b].level = 0;
   }
   const size_t outb = (size_t)nb * B;
   if (int rc = ensure(g, g->aux, al(n) + al(outb) + 2 * al((size_t)nb * 4) + al((size_t)nb * 32))) return rc;
@@ -488,7 +488,7 @@ int zgpu_decompress(zgpu_t* g, const uint8_t* comp, const uint32_t* comp_len, co
     tab[b].raw_len = raw_len[b];
     tab[b].comp_len = comp_len[b];
     tab[b].method = method[b];
-    tab[b].pad = 0;
+    tab[b].level = 0;
     co += comp_len[b];
     ro += raw_len[b];
   }
@@ -543,3 +543,4 @@ int zgpu_crc32(zgpu_t* g, const uint8_t* data, uint64_t n, uint32_t blog, uint32
 }  // extern "C"
 
 #include "plan.hip"
+#include "serve.hip"

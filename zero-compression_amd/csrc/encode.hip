@@ -50,6 +50,12 @@ __device__ inline uint32_t prefix_len(const uint8_t* inb, uint32_t a, uint32_t b
 
 __device__ inline uint32_t bitlen(uint32_t d) { return 31u - (uint32_t)__builtin_clz(d); }
 
+// level 1..9 -> (NCANDL, LENCAP): neighbours compared and bytes compared before the extension (src/gpu_codec.h:66-70)
+__host__ __device__ inline void level_params(uint32_t level, uint32_t& ncand, uint32_t& lencap) {
+  ncand = level <= 2 ? 2u : level <= 6 ? 4u : 8u;
+  lencap = level <= 6 ? 16u : 32u;
+}
+
 // ---------------------------------------------------------------- 1. hash, count, scatter (one workgroup per block)
 // The key of a position is the top `keybits` bits of its hash; buckets are its top `hbits` bits. srt[b*B + slot] =
 // (the key's other keybits - hbits bits) << blog | position, grouped by bucket; bstart[b*(HB+1) + h] = the first slot
@@ -287,6 +293,7 @@ __global__ __launch_bounds__(256) void k_enc_match(const Block* __restrict__ blo
   const Block blk = blocks[b];
   const uint32_t n = blk.raw_len;
   if (i >= n) return;
+  if (blk.level) level_params(blk.level, ncand, lencap);
   const uint32_t count = n >= 4 ? n - 3 : 0;
   uint32_t* mb = match + ((size_t)b << blog);
   if (i >= count) { mb[i] = 0; return; }  // (the last 3 positions are never in srt: exactly these slots)
@@ -380,6 +387,7 @@ __global__ __launch_bounds__(256) void k_enc_parse(const Block* __restrict__ blo
   const uint8_t* inb = in + blk.src;
   const uint32_t n = blk.raw_len;
   uint32_t* m = lazy + ((size_t)b << blog);
+  if (blk.level) { uint32_t nc; level_params(blk.level, nc, lencap); }
   for (uint32_t i = k; i < B / 32u; i += 256) bm[i] = 0;
   __syncthreads();
   if (k < walkers) {

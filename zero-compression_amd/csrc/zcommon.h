@@ -26,14 +26,15 @@ __host__ __device__ inline Geo geo_of(uint32_t blog) {
 }
 
 // One entry per block of a launch (32 bytes). For decoding: src/comp_len/method locate the block's stored bytes,
-// dst/raw_len where its raw bytes go. For encoding: src/raw_len locate the raw bytes; dst is unused.
+// dst/raw_len where its raw bytes go. For encoding: src/raw_len locate the raw bytes, level (1..9) overrides the
+// launch's level for this block when not 0; dst is unused.
 struct Block {
   uint64_t src;
   uint64_t dst;
   uint32_t raw_len;
   uint32_t comp_len;
   uint32_t method;
-  uint32_t pad;
+  uint32_t level;
 };
 
 // per-block decoder error bits (err[b]); the host reports any as a damaged block

@@ -35,10 +35,11 @@ EXPORTS = (
     "zgpu_set_profile", "zgpu_stage_times",
     "zgpu_plan_create", "zgpu_plan_destroy", "zgpu_plan_slab_bytes", "zgpu_plan_slab_offsets", "zgpu_plan_classes",
     "zgpu_plan_encode", "zgpu_plan_set_stored", "zgpu_plan_decode", "zgpu_plan_tune", "zgpu_plan_use_graphs",
+    "zgpu_serve_bench",
 )
 
 BLOCK_DTYPE = np.dtype([("src", "<u8"), ("dst", "<u8"), ("raw_len", "<u4"), ("comp_len", "<u4"),
-                        ("method", "<u4"), ("pad", "<u4")])
+                        ("method", "<u4"), ("level", "<u4")])
 
 
 class ZeroError(RuntimeError):
@@ -94,6 +95,8 @@ def load_library(path=LIB_PATH):
         "zgpu_plan_encode": (i32, [vp, vp, i32, vp, vp]), "zgpu_plan_set_stored": (i32, [vp, vp, vp, vp]),
         "zgpu_plan_decode": (i32, [vp, vp, vp, vp]), "zgpu_plan_tune": (i32, [vp, vp, i32, vp, vp]),
         "zgpu_plan_use_graphs": (i32, [vp, i32]),
+        "zgpu_serve_bench": (i32, [vp, ctypes.POINTER(ServeCfg), vp, ctypes.c_uint64, vp, ctypes.c_uint64,
+                                   ctypes.POINTER(ServeReport)]),
         "zgpu_stream_create": (i32, [vp, ctypes.POINTER(vp)]), "zgpu_stream_destroy": (i32, [vp, vp]),
         "zgpu_stream_sync": (i32, [vp, vp]),
         "zgpu_event_create": (i32, [vp, ctypes.POINTER(vp)]), "zgpu_event_destroy": (i32, [vp, vp]),
@@ -121,6 +124,21 @@ def load_library(path=LIB_PATH):
 def _ptr(a):
     """Address of a numpy array's data (contiguous)."""
     return ctypes.c_void_p(a.ctypes.data)
+
+
+class ServeCfg(ctypes.Structure):
+    _fields_ = [("offered_GBps", ctypes.c_double), ("seconds", ctypes.c_double), ("compress_share", ctypes.c_double),
+                ("random_share", ctypes.c_double), ("size_median", ctypes.c_double), ("size_sigma", ctypes.c_double),
+                ("size_max", ctypes.c_uint64), ("hbm_budget", ctypes.c_uint64), ("block_log", ctypes.c_uint32),
+                ("slots", ctypes.c_int), ("seed", ctypes.c_uint64), ("alone_checks", ctypes.c_uint32)]
+
+
+class ServeReport(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("requests", "compress_requests", "decompress_requests", "bytes")] + \
+               [(n, ctypes.c_double) for n in ("p50_ms", "p99_ms", "max_ms", "mean_ms", "wall_s", "sustained_GBps")] + \
+               [("batches", ctypes.c_uint64), ("mean_batch_requests", ctypes.c_double)] + \
+               [(n, ctypes.c_uint64) for n in ("hbm_bytes", "slot_input_bytes", "mismatches", "alone_checked",
+                                               "alone_mismatches")]
 
 
 class DeviceBuffer:
@@ -253,6 +271,19 @@ class Gpu:
 
     def sync(self):
         self._check(self.lib.zgpu_sync(self.h))
+
+    def serve_bench(self, lz_pool, random_pool, offered_GBps=20.0, seconds=1.0, compress_share=0.7,
+                    random_share=0.1, size_median=256 * 1024, size_sigma=1.5, size_max=256 << 20,
+                    hbm_budget=64 << 30, block_log=18, slots=3, seed=5, alone_checks=64):
+        """The serving loop under an open-loop synthetic load (zgpu_serve_bench); returns its report as a dict."""
+        cfg = ServeCfg(offered_GBps, seconds, compress_share, random_share, size_median, size_sigma, size_max,
+                       hbm_budget, block_log, slots, seed, alone_checks)
+        rep = ServeReport()
+        lz = np.ascontiguousarray(lz_pool, dtype=np.uint8)
+        rn = np.ascontiguousarray(random_pool, dtype=np.uint8)
+        self._check(self.lib.zgpu_serve_bench(self.h, ctypes.byref(cfg), _ptr(lz), lz.size, _ptr(rn), rn.size,
+                                              ctypes.byref(rep)))
+        return {n: getattr(rep, n) for n, _ in ServeReport._fields_}
 
     def copy(self, dst, src, nbytes, stream=None):
         """device-to-device copy on the codec's stream (a plain kernel when both addresses are 16-byte aligned)"""
