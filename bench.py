@@ -14,6 +14,11 @@ from the reference's headers) on a bounded sample of the same input, on this mac
 ``--workload uneven`` measures the batched form instead: 200,000 files with lognormal sizes (median 8 KiB, sigma 2,
 clipped to 64 B .. 64 MiB) cut from the seed-1 LZ stream, level 5, 2^18 blocks, through one size-class plan
 (autotuned, replayed as a hipGraph), against the plain path called once per file on a sample of the same files.
+
+``--workload serve`` runs the serving loop under an open-loop load: Poisson arrivals at 20 GB/s of raw bytes for one
+second, 70 % compress / 30 % decompress, sizes lognormal (median 256 KiB, sigma 1.5, at most 256 MiB), seed-1 LZ and
+random bytes 9 : 1, levels uniform over 1 / 5 / 9, a 64 GiB HBM budget; latency from arrival to the result in host
+memory. The same load at 8 GB/s is reported beside it.
 """
 import argparse
 import json
@@ -203,6 +208,39 @@ def bench_uneven(a, gpu):
     print(json.dumps(line), flush=True)
 
 
+def bench_serve(a, gpu):
+    """The serving loop at the configured offered load and at a lighter one (one JSON line)."""
+    import pyzero
+
+    t0 = time.time()
+    lz = pyzero.gen_lz(1, 2 << 30)
+    rnd = pyzero.gen_random(2, 256 << 20)
+    log(f"serve: pools generated in {time.time() - t0:.1f} s")
+    runs = {}
+    for offered in (a.offered, 8.0):
+        t0 = time.time()
+        r = gpu.serve_bench(lz, rnd, offered_GBps=offered, seconds=a.seconds, block_log=a.block_log,
+                            hbm_budget=64 << 30, slots=a.slots, seed=5, alone_checks=64)
+        log(f"serve at {offered} GB/s offered: {r['requests']} requests, sustained {r['sustained_GBps']:.2f} GB/s, "
+            f"p50 {r['p50_ms']:.2f} ms, p99 {r['p99_ms']:.2f} ms, max {r['max_ms']:.1f} ms, {r['batches']} batches "
+            f"(mean {r['mean_batch_requests']:.1f} requests), mismatches {r['mismatches']}, alone vs mix "
+            f"{r['alone_mismatches']}/{r['alone_checked']} ({time.time() - t0:.1f} s)")
+        runs[offered] = r
+    main_run = runs[a.offered]
+    line = {
+        "metric": "serving p50/p99 latency (arrival to result in host memory) and sustained GB/s",
+        "value": round(main_run["sustained_GBps"], 3), "unit": "GB/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": round(main_run["wall_s"] * 1e3, 1), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u8", "data": "synthetic (seeded generators; no dataset)",
+        "config": {"workload": f"open-loop Poisson arrivals at {a.offered} GB/s for {a.seconds} s, 70 % compress, "
+                               "lognormal sizes (median 256 KiB, sigma 1.5, <= 256 MiB), LZ : random 9 : 1, levels "
+                               f"1/5/9, 2^{a.block_log} blocks, 64 GiB HBM budget, {a.slots} batch slots"},
+        "runs": {str(k): {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                 for k, v in runs.items()},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -212,7 +250,10 @@ def main():
     ap.add_argument("--level", type=int, default=5)
     ap.add_argument("--block-log", type=int, default=18)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["plain", "uneven"], default="plain")
+    ap.add_argument("--workload", choices=["plain", "uneven", "serve"], default="plain")
+    ap.add_argument("--offered", type=float, default=20.0, help="serve workload: offered load in GB/s")
+    ap.add_argument("--seconds", type=float, default=1.0, help="serve workload: arrival window")
+    ap.add_argument("--slots", type=int, default=3, help="serve workload: batches in flight")
     ap.add_argument("--files", type=int, default=200000, help="uneven workload: number of files")
     ap.add_argument("--sample-files", type=int, default=2000, help="uneven workload: files timed per-file")
     ap.add_argument("--streams", type=int, default=4, help="uneven workload: streams of the plan")
@@ -230,8 +271,8 @@ def main():
     gpu = pyzero.Gpu(0)
     props = gpu.props()
     log(f"device {props['name']} {props['arch']} {props['cu_count']} CUs")
-    if a.workload == "uneven":
-        bench_uneven(a, gpu)
+    if a.workload in ("uneven", "serve"):
+        (bench_uneven if a.workload == "uneven" else bench_serve)(a, gpu)
         gpu.close()
         return
     t0 = time.time()
