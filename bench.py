@@ -19,6 +19,9 @@ clipped to 64 B .. 64 MiB) cut from the seed-1 LZ stream, level 5, 2^18 blocks, 
 second, 70 % compress / 30 % decompress, sizes lognormal (median 256 KiB, sigma 1.5, at most 256 MiB), seed-1 LZ and
 random bytes 9 : 1, levels uniform over 1 / 5 / 9, a 64 GiB HBM budget; latency from arrival to the result in host
 memory. The same load at 8 GB/s is reported beside it.
+
+``--workload sweep`` crosses block sizes 2^16 / 2^18 / 2^20 with levels 1 / 5 / 9 over 2 GiB of executable-like
+pages (seed 3): ratio, encode and decode GB/s per point, every round trip checked.
 """
 import argparse
 import json
@@ -241,6 +244,86 @@ def bench_serve(a, gpu):
     print(json.dumps(line), flush=True)
 
 
+def bench_sweep(a, gpu):
+    """Block size x level over the executable-like workload (one JSON line with every point)."""
+    import pyzero
+
+    n = int(a.gib * (1 << 30)) if a.gib != 4.0 else 2 << 30
+    t0 = time.time()
+    data = pyzero.gen_exec(3, n)
+    log(f"sweep: {n >> 20} MiB of executable-like pages generated in {time.time() - t0:.1f} s")
+    d_in = gpu.alloc(n)
+    d_in.upload(data)
+    ev0, ev1 = gpu.event(), gpu.event()
+    points = []
+    for bl in (16, 18, 20):
+        B = 1 << bl
+        nb = n // B
+        tab = pyzero.block_table([B] * nb, src=np.arange(nb, dtype=np.uint64) * B,
+                                 dst=np.arange(nb, dtype=np.uint64) * B)
+        d_tab = gpu.alloc(tab.nbytes)
+        d_tab.upload(tab)
+        batch = max(1, (1 << 30) >> bl)
+        slabs = gpu.alloc(n)
+        lens = gpu.alloc(nb * 4)
+        for level in (1, 5, 9):
+            def enc():
+                for b0 in range(0, nb, batch):
+                    k = min(batch, nb - b0)
+                    gpu.encode_blocks(d_in.ptr, d_tab.addr(b0 * 32), k, bl, level, slabs.addr(b0 << bl),
+                                      lens.addr(b0 * 4))
+            enc()
+            gpu.sync()
+            gpu.record(ev0)
+            for _ in range(a.steps):
+                enc()
+            gpu.record(ev1)
+            gpu.sync()
+            enc_ms = gpu.elapsed_ms(ev0, ev1) / a.steps
+            ln = lens.download(dtype=np.uint32).copy()
+            st = ln == pyzero.STORED
+            clen = np.where(st, B, ln).astype(np.uint64)
+            sl = slabs.download()
+            blob = np.concatenate([data[b * B:(b + 1) * B] if st[b] else sl[b * B: b * B + int(ln[b])]
+                                   for b in range(nb)])
+            del sl
+            dt = pyzero.block_table([B] * nb, comp_lens=clen, methods=np.where(st, 0, 1))
+            d_dt = gpu.alloc(dt.nbytes)
+            d_dt.upload(dt)
+            d_comp = gpu.alloc(blob.size)
+            d_comp.upload(blob)
+            d_out = gpu.alloc(n)
+            d_err = gpu.alloc(nb * 4)
+            gpu.decode_blocks(d_comp.ptr, d_dt.ptr, nb, bl, d_out.ptr, d_err.ptr)
+            gpu.sync()
+            gpu.record(ev0)
+            for _ in range(a.steps):
+                gpu.decode_blocks(d_comp.ptr, d_dt.ptr, nb, bl, d_out.ptr, d_err.ptr)
+            gpu.record(ev1)
+            gpu.sync()
+            dec_ms = gpu.elapsed_ms(ev0, ev1) / a.steps
+            ok = not d_err.download(dtype=np.uint32).any() and np.array_equal(d_out.download(), data)
+            p = {"block_log": bl, "level": level, "ratio": round(blob.size / n, 5),
+                 "compress_GBps": round(n / (enc_ms * 1e-3) / 1e9, 2), "decompress_GBps": round(n / (dec_ms * 1e-3) / 1e9, 2),
+                 "stored_blocks": int(st.sum()), "roundtrip_identical": bool(ok)}
+            log(f"sweep {p}")
+            points.append(p)
+            for buf in (d_dt, d_comp, d_out, d_err):
+                buf.free()
+        for buf in (d_tab, slabs, lens):
+            buf.free()
+    l5 = [p for p in points if p["block_log"] == 18 and p["level"] == 5][0]
+    line = {
+        "metric": "block/level sweep: ratio and compress/decompress GB/s per point", "value": l5["decompress_GBps"],
+        "unit": "GB/s", "n_gpus": 1, "steps": a.steps, "warmup": 1, "ms_per_step": None, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic (seeded generators; no dataset)",
+        "config": {"workload": f"{n >> 20} MiB executable-like pages (seed 3), blocks 2^16/2^18/2^20 x levels 1/5/9; "
+                               "value = decode GB/s at 2^18, level 5"},
+        "points": points,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -250,7 +333,7 @@ def main():
     ap.add_argument("--level", type=int, default=5)
     ap.add_argument("--block-log", type=int, default=18)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["plain", "uneven", "serve"], default="plain")
+    ap.add_argument("--workload", choices=["plain", "uneven", "serve", "sweep"], default="plain")
     ap.add_argument("--offered", type=float, default=20.0, help="serve workload: offered load in GB/s")
     ap.add_argument("--seconds", type=float, default=1.0, help="serve workload: arrival window")
     ap.add_argument("--slots", type=int, default=3, help="serve workload: batches in flight")
@@ -271,8 +354,8 @@ def main():
     gpu = pyzero.Gpu(0)
     props = gpu.props()
     log(f"device {props['name']} {props['arch']} {props['cu_count']} CUs")
-    if a.workload in ("uneven", "serve"):
-        (bench_uneven if a.workload == "uneven" else bench_serve)(a, gpu)
+    if a.workload != "plain":
+        {"uneven": bench_uneven, "serve": bench_serve, "sweep": bench_sweep}[a.workload](a, gpu)
         gpu.close()
         return
     t0 = time.time()
