@@ -62,6 +62,21 @@ __device__ inline Sections read_sections(const uint8_t* p, uint32_t clen, uint32
   return S;
 }
 
+struct U128W {
+  uint64_t lo, hi;
+};
+// a wavefront copies n <= 2048 bytes from LDS to d (16 bytes per lane per step)
+__device__ inline void wave_copy_lds(uint8_t* d, const uint8_t* s, uint32_t n) {
+  const uint32_t lane = lane_id();
+  const uint32_t full = n & ~15u;
+  for (uint32_t i = lane * 16; i < full; i += 64 * 16) {
+    U128W v;
+    __builtin_memcpy(&v, s + i, 16);
+    __builtin_memcpy(d + i, &v, 16);
+  }
+  if (full + lane < n) d[full + lane] = s[full + lane];
+}
+
 // ---------------------------------------------------------------- the streams: one wavefront per (block, stream)
 __global__ __launch_bounds__(64) void k_dec_streams(const Block* __restrict__ blocks, const uint8_t* __restrict__ comp,
                                                     uint32_t blog, uint8_t* __restrict__ lit, uint8_t* __restrict__ tok,
@@ -97,6 +112,7 @@ __global__ __launch_bounds__(64) void k_dec_streams(const Block* __restrict__ bl
   __shared__ uint32_t fc[256];       // freq | cum << 16
   __shared__ uint8_t slot[4096];     // slot -> symbol
   __shared__ uint32_t lstart[65];    // lane data offsets in the body
+  __shared__ uint8_t stg[2048];      // decoded symbols on their way out
 
   // frequencies of the present symbols, in symbol order; lane k owns symbols 4k..4k+3
   uint32_t np = lane < 32 ? (uint32_t)__popc(body[lane]) : 0u;
@@ -148,43 +164,76 @@ __global__ __launch_bounds__(64) void k_dec_streams(const Block* __restrict__ bl
   }
   __syncthreads();
   // 64 lanes decode side by side: lane j holds symbols j, j+64, ... A step needs 0, 1 or 2 more bytes of the lane
-  // (x >= 2^11 after the step: 2^15 <= x < 2^23 takes one, x < 2^15 two). Each lane keeps a window w of nw <= 8 of
-  // its next bytes and the 8 after them, requested when w was last refilled, so a step never waits on a load it
-  // just issued. Reads stop at the payload's end; a lane that consumes more than its length is damaged.
+  // (x >= 2^11 after the step: 2^15 <= x < 2^23 takes one, x < 2^15 two). Steps go 8 at a time over a 16-byte window
+  // of the lane's next bytes (8 steps take at most 16); the 16 bytes after the window were requested when the window
+  // was set, 8 steps earlier, so no step waits on memory. The symbols collect in LDS and leave 2048 at a time in
+  // 16-byte stores (the vector memory counter is shared by loads and stores: a byte store per step would make every
+  // wait for a load also wait for the stores). Reads stop at the payload's end; a lane that consumes more bytes than
+  // its length is damaged.
   const uint8_t* lp = body + lstart[lane];
   const uint32_t lnl = lstart[lane + 1] - lstart[lane];
   const uint8_t* lim = p + blk.comp_len;
-  uint32_t x = ld32u(lp);
-  const uint8_t* rd = lp + 4;
-  auto fetch8 = [&]() {
-    const uint64_t v = load8n(rd, rd < lim ? (uint32_t)min((size_t)8, (size_t)(lim - rd)) : 0u);
-    rd += 8;
-    return v;
-  };
-  uint64_t w = fetch8();
-  uint64_t nxt = fetch8();
-  uint32_t nw = 8, used = 4;
-  for (uint32_t i = lane; i < n; i += 64) {
-    const uint32_t sl = x & 4095u;
-    const uint32_t sy = slot[sl];
-    const uint32_t e = fc[sy];
-    dest[i] = (uint8_t)sy;
-    x = (e & 0xFFFFu) * (x >> 12) + sl - (e >> 16);
-    const uint32_t need = (x < (1u << 15) ? 1u : 0u) + (x < (1u << 23) ? 1u : 0u);
-    const uint64_t cat = nw >= 2 ? w : nw == 1 ? (w & 0xFFull) | (nxt << 8) : nxt;  // the next two bytes
-    const uint32_t b0 = (uint32_t)cat & 0xFFu, b1 = (uint32_t)(cat >> 8) & 0xFFu;
-    x = need == 0 ? x : need == 1 ? (x << 8) | b0 : (x << 16) | (b0 << 8) | b1;
-    used += need;
-    if (need > nw) {  // w is used up: go on in nxt and request the 8 bytes after it
-      w = nxt >> (8u * (need - nw));
-      nw += 8u - need;
-      nxt = fetch8();
+  auto load16 = [&](uint32_t at, uint32_t (&v)[4]) {
+    const uint8_t* q = lp + at;
+    if (q + 16 <= lim) {
+      __builtin_memcpy(v, q, 16);
     } else {
-      w >>= 8u * need;
-      nw -= need;
+      for (uint32_t k = 0; k < 4; ++k) v[k] = 0;
+      for (uint32_t k = 0; k < 16; ++k)
+        if (q + k < lim) v[k >> 2] |= (uint32_t)q[k] << (8 * (k & 3));
     }
+  };
+  uint32_t x = ld32u(lp);
+  uint32_t pos = 4;  // bytes of the lane consumed; the window holds bytes [pos, pos + 16)
+  uint32_t cw[4], nx[4];  // the window; nx: the 16 bytes after it
+  load16(pos, cw);
+  load16(pos + 16, nx);
+  const uint32_t nsteps = (n + 63) >> 6;
+  for (uint32_t seg = 0; seg < nsteps; seg += 32) {
+    const uint32_t seg_end = min(seg + 32u, nsteps);
+    for (uint32_t b0 = seg; b0 < seg_end; b0 += 8) {
+      uint32_t o = 0;  // bytes consumed in these 8 steps (at most 16)
+#pragma unroll
+      for (uint32_t k = 0; k < 8; ++k) {
+        if (b0 + k < seg_end) {
+          const uint32_t st = b0 + k;
+          const bool valid = st * 64 + lane < n;
+          const uint32_t sl = x & 4095u;
+          const uint32_t sy = slot[sl];
+          const uint32_t e = fc[sy];
+          stg[(st & 31u) * 64 + lane] = (uint8_t)sy;
+          const uint32_t xs = (e & 0xFFFFu) * (x >> 12) + sl - (e >> 16);
+          const uint32_t need = (xs < (1u << 15) ? 1u : 0u) + (xs < (1u << 23) ? 1u : 0u);
+          // the next two bytes, at window offset o <= 14
+          const uint32_t q = o >> 2;
+          const uint32_t lo = q == 0 ? cw[0] : q == 1 ? cw[1] : q == 2 ? cw[2] : cw[3];
+          const uint32_t hi = q == 0 ? cw[1] : q == 1 ? cw[2] : q == 2 ? cw[3] : 0u;
+          const uint32_t w2 = __builtin_amdgcn_alignbyte(hi, lo, o & 3u);
+          const uint32_t v = (w2 & 0xFFu) << 8 | ((w2 >> 8) & 0xFFu);  // first byte high
+          const uint32_t xn = (xs << (8u * need)) | (v >> (16u - 8u * need));
+          x = valid ? xn : x;
+          o += valid ? need : 0u;
+        }
+      }
+      // the window moves on by o bytes: bytes [o, o + 16) of cw then nx
+      const uint32_t q = o >> 2, r = o & 3u;
+      const uint32_t e0 = q == 0 ? cw[0] : q == 1 ? cw[1] : q == 2 ? cw[2] : q == 3 ? cw[3] : nx[0];
+      const uint32_t e1 = q == 0 ? cw[1] : q == 1 ? cw[2] : q == 2 ? cw[3] : q == 3 ? nx[0] : nx[1];
+      const uint32_t e2 = q == 0 ? cw[2] : q == 1 ? cw[3] : q == 2 ? nx[0] : q == 3 ? nx[1] : nx[2];
+      const uint32_t e3 = q == 0 ? cw[3] : q == 1 ? nx[0] : q == 2 ? nx[1] : q == 3 ? nx[2] : nx[3];
+      const uint32_t e4 = q == 0 ? nx[0] : q == 1 ? nx[1] : q == 2 ? nx[2] : q == 3 ? nx[3] : 0u;
+      cw[0] = __builtin_amdgcn_alignbyte(e1, e0, r);
+      cw[1] = __builtin_amdgcn_alignbyte(e2, e1, r);
+      cw[2] = __builtin_amdgcn_alignbyte(e3, e2, r);
+      cw[3] = __builtin_amdgcn_alignbyte(e4, e3, r);
+      pos += o;
+      load16(pos + 16, nx);
+    }
+    wave_fence();  // 2048 symbols (or the last of them) to their place
+    wave_copy_lds(dest + seg * 64, stg, min(n - seg * 64, 2048u));
+    wave_fence();
   }
-  if (ballot(used > lnl) && lane == 0) atomicOr(&err[b], E_LANE);
+  if (ballot(pos > lnl) && lane == 0) atomicOr(&err[b], E_LANE);
 }
 
 // ---------------------------------------------------------------- the LZ replay: one wavefront per block
