@@ -65,7 +65,7 @@ __device__ inline Sections read_sections(const uint8_t* p, uint32_t clen, uint32
 struct U128W {
   uint64_t lo, hi;
 };
-// a wavefront copies n <= 2048 bytes from LDS to d (16 bytes per lane per step)
+// a wavefront copies n bytes from LDS to d (16 bytes per lane per step)
 __device__ inline void wave_copy_lds(uint8_t* d, const uint8_t* s, uint32_t n) {
   const uint32_t lane = lane_id();
   const uint32_t full = n & ~15u;
@@ -75,6 +75,50 @@ __device__ inline void wave_copy_lds(uint8_t* d, const uint8_t* s, uint32_t n) {
     __builtin_memcpy(d + i, &v, 16);
   }
   if (full + lane < n) d[full + lane] = s[full + lane];
+}
+
+// the 16 bytes of a lane at lp + at, requested without a branch: the load starts at min(lp + at, lim - 16) (lim - 16
+// is inside the payload, which is at least 20 bytes), and d (0..16) is how far the wanted bytes lie past that start;
+// shift16 then drops those d bytes and fills zeros, for the bytes past lim
+__device__ inline void load16(const uint8_t* lp, uint32_t at, const uint8_t* lim, uint32_t& v0, uint32_t& v1,
+                              uint32_t& v2, uint32_t& v3, uint32_t& d) {
+  const uint8_t* q = lp + at;
+  const uint8_t* a = q + 16 <= lim ? q : lim - 16;
+  d = (uint32_t)min((uint64_t)(q - a), (uint64_t)16);
+  uint32_t v[4];
+  __builtin_memcpy(v, a, 16);
+  v0 = v[0]; v1 = v[1]; v2 = v[2]; v3 = v[3];
+}
+__device__ inline void shift16(uint32_t& v0, uint32_t& v1, uint32_t& v2, uint32_t& v3, uint32_t d) {
+  const uint32_t q = d >> 2, r = d & 3u;
+  const uint32_t e0 = q == 0 ? v0 : q == 1 ? v1 : q == 2 ? v2 : q == 3 ? v3 : 0u;
+  const uint32_t e1 = q == 0 ? v1 : q == 1 ? v2 : q == 2 ? v3 : 0u;
+  const uint32_t e2 = q == 0 ? v2 : q == 1 ? v3 : 0u;
+  const uint32_t e3 = q == 0 ? v3 : 0u;
+  v0 = __builtin_amdgcn_alignbyte(e1, e0, r);
+  v1 = __builtin_amdgcn_alignbyte(e2, e1, r);
+  v2 = __builtin_amdgcn_alignbyte(e3, e2, r);
+  v3 = __builtin_amdgcn_alignbyte(0u, e3, r);
+}
+
+// one rANS decoding step of a lane: the symbol of state x (slot table, then its freq | cum << 16) goes to *out; the
+// renormalisation takes 0, 1 or 2 bytes from the window c0..c3 at byte offset o (at most 14), and o advances
+__device__ inline void rans_step(const uint8_t* slot, const uint32_t* fc, uint8_t* out, bool valid, uint32_t c0,
+                                 uint32_t c1, uint32_t c2, uint32_t c3, uint32_t& x, uint32_t& o) {
+  const uint32_t sl = x & 4095u;
+  const uint32_t sy = slot[sl];
+  const uint32_t e = fc[sy];
+  *out = (uint8_t)sy;
+  const uint32_t xs = (e & 0xFFFFu) * (x >> 12) + sl - (e >> 16);
+  const uint32_t need = (xs < (1u << 15) ? 1u : 0u) + (xs < (1u << 23) ? 1u : 0u);
+  const uint32_t q = o >> 2;
+  const uint32_t lo = q == 0 ? c0 : q == 1 ? c1 : q == 2 ? c2 : c3;
+  const uint32_t hi = q == 0 ? c1 : q == 1 ? c2 : q == 2 ? c3 : 0u;
+  const uint32_t w2 = __builtin_amdgcn_alignbyte(hi, lo, o & 3u);
+  const uint32_t v = (w2 & 0xFFu) << 8 | ((w2 >> 8) & 0xFFu);  // first byte high
+  const uint32_t xn = (xs << (8u * need)) | (v >> (16u - 8u * need));
+  x = valid ? xn : x;
+  o += valid ? need : 0u;
 }
 
 // ---------------------------------------------------------------- the streams: one wavefront per (block, stream)
@@ -111,8 +155,7 @@ __global__ __launch_bounds__(64) void k_dec_streams(const Block* __restrict__ bl
 
   __shared__ uint32_t fc[256];       // freq | cum << 16
   __shared__ uint8_t slot[4096];     // slot -> symbol
-  __shared__ uint32_t lstart[65];    // lane data offsets in the body
-  __shared__ uint8_t stg[2048];      // decoded symbols on their way out
+  __shared__ uint8_t stg[512];       // decoded symbols on their way out
 
   // frequencies of the present symbols, in symbol order; lane k owns symbols 4k..4k+3
   uint32_t np = lane < 32 ? (uint32_t)__popc(body[lane]) : 0u;
@@ -144,8 +187,6 @@ __global__ __launch_bounds__(64) void k_dec_streams(const Block* __restrict__ bl
   if (ballot(ll < 4 || ll > len - data0 || len > (1u << 25))) { if (lane == 0) atomicOr(&err[b], E_LANE); return; }
   const uint32_t lend = wave_scan_incl(ll);
   if (readlane(lend, 63) > len - data0) { if (lane == 0) atomicOr(&err[b], E_LANE); return; }
-  lstart[lane] = data0 + lend - ll;
-  if (lane == 63) lstart[64] = data0 + lend;
   __syncthreads();
   // slot table: lane j fills slots 64j..64j+63
   {
@@ -166,71 +207,51 @@ __global__ __launch_bounds__(64) void k_dec_streams(const Block* __restrict__ bl
   // 64 lanes decode side by side: lane j holds symbols j, j+64, ... A step needs 0, 1 or 2 more bytes of the lane
   // (x >= 2^11 after the step: 2^15 <= x < 2^23 takes one, x < 2^15 two). Steps go 8 at a time over a 16-byte window
   // of the lane's next bytes (8 steps take at most 16); the 16 bytes after the window were requested when the window
-  // was set, 8 steps earlier, so no step waits on memory. The symbols collect in LDS and leave 2048 at a time in
+  // was set, 8 steps earlier, so no step waits on memory. The symbols collect in LDS and leave 512 at a time in
   // 16-byte stores (the vector memory counter is shared by loads and stores: a byte store per step would make every
   // wait for a load also wait for the stores). Reads stop at the payload's end; a lane that consumes more bytes than
   // its length is damaged.
-  const uint8_t* lp = body + lstart[lane];
-  const uint32_t lnl = lstart[lane + 1] - lstart[lane];
+  const uint8_t* lp = body + data0 + lend - ll;  // this lane's bytes
+  const uint32_t lnl = ll;
   const uint8_t* lim = p + blk.comp_len;
-  auto load16 = [&](uint32_t at, uint32_t (&v)[4]) {
-    const uint8_t* q = lp + at;
-    if (q + 16 <= lim) {
-      __builtin_memcpy(v, q, 16);
-    } else {
-      for (uint32_t k = 0; k < 4; ++k) v[k] = 0;
-      for (uint32_t k = 0; k < 16; ++k)
-        if (q + k < lim) v[k >> 2] |= (uint32_t)q[k] << (8 * (k & 3));
-    }
-  };
   uint32_t x = ld32u(lp);
-  uint32_t pos = 4;  // bytes of the lane consumed; the window holds bytes [pos, pos + 16)
-  uint32_t cw[4], nx[4];  // the window; nx: the 16 bytes after it
-  load16(pos, cw);
-  load16(pos + 16, nx);
-  const uint32_t nsteps = (n + 63) >> 6;
-  for (uint32_t seg = 0; seg < nsteps; seg += 32) {
-    const uint32_t seg_end = min(seg + 32u, nsteps);
-    for (uint32_t b0 = seg; b0 < seg_end; b0 += 8) {
-      uint32_t o = 0;  // bytes consumed in these 8 steps (at most 16)
+  uint32_t pos = 4;  // bytes of the lane consumed; the window c0..c3 holds bytes [pos, pos + 16)
+  uint32_t c0, c1, c2, c3, n0, n1, n2, n3, cd, nd;  // (n0..n3: the 16 bytes after the window, shifted by nd)
+  load16(lp, pos, lim, c0, c1, c2, c3, cd);
+  shift16(c0, c1, c2, c3, cd);
+  load16(lp, pos + 16, lim, n0, n1, n2, n3, nd);
+  const uint32_t nsteps = (n + 63) >> 6, nfull = n >> 6;  // (every lane has a symbol in the first nfull steps)
+  uint32_t b0 = 0;
+  for (; b0 + 8 <= nfull; b0 += 8) {
+    uint32_t o = 0;
 #pragma unroll
-      for (uint32_t k = 0; k < 8; ++k) {
-        if (b0 + k < seg_end) {
-          const uint32_t st = b0 + k;
-          const bool valid = st * 64 + lane < n;
-          const uint32_t sl = x & 4095u;
-          const uint32_t sy = slot[sl];
-          const uint32_t e = fc[sy];
-          stg[(st & 31u) * 64 + lane] = (uint8_t)sy;
-          const uint32_t xs = (e & 0xFFFFu) * (x >> 12) + sl - (e >> 16);
-          const uint32_t need = (xs < (1u << 15) ? 1u : 0u) + (xs < (1u << 23) ? 1u : 0u);
-          // the next two bytes, at window offset o <= 14
-          const uint32_t q = o >> 2;
-          const uint32_t lo = q == 0 ? cw[0] : q == 1 ? cw[1] : q == 2 ? cw[2] : cw[3];
-          const uint32_t hi = q == 0 ? cw[1] : q == 1 ? cw[2] : q == 2 ? cw[3] : 0u;
-          const uint32_t w2 = __builtin_amdgcn_alignbyte(hi, lo, o & 3u);
-          const uint32_t v = (w2 & 0xFFu) << 8 | ((w2 >> 8) & 0xFFu);  // first byte high
-          const uint32_t xn = (xs << (8u * need)) | (v >> (16u - 8u * need));
-          x = valid ? xn : x;
-          o += valid ? need : 0u;
-        }
-      }
-      // the window moves on by o bytes: bytes [o, o + 16) of cw then nx
-      const uint32_t q = o >> 2, r = o & 3u;
-      const uint32_t e0 = q == 0 ? cw[0] : q == 1 ? cw[1] : q == 2 ? cw[2] : q == 3 ? cw[3] : nx[0];
-      const uint32_t e1 = q == 0 ? cw[1] : q == 1 ? cw[2] : q == 2 ? cw[3] : q == 3 ? nx[0] : nx[1];
-      const uint32_t e2 = q == 0 ? cw[2] : q == 1 ? cw[3] : q == 2 ? nx[0] : q == 3 ? nx[1] : nx[2];
-      const uint32_t e3 = q == 0 ? cw[3] : q == 1 ? nx[0] : q == 2 ? nx[1] : q == 3 ? nx[2] : nx[3];
-      const uint32_t e4 = q == 0 ? nx[0] : q == 1 ? nx[1] : q == 2 ? nx[2] : q == 3 ? nx[3] : 0u;
-      cw[0] = __builtin_amdgcn_alignbyte(e1, e0, r);
-      cw[1] = __builtin_amdgcn_alignbyte(e2, e1, r);
-      cw[2] = __builtin_amdgcn_alignbyte(e3, e2, r);
-      cw[3] = __builtin_amdgcn_alignbyte(e4, e3, r);
-      pos += o;
-      load16(pos + 16, nx);
-    }
-    wave_fence();  // 2048 symbols (or the last of them) to their place
-    wave_copy_lds(dest + seg * 64, stg, min(n - seg * 64, 2048u));
+    for (uint32_t k = 0; k < 8; ++k) rans_step(slot, fc, stg + k * 64 + lane, true, c0, c1, c2, c3, x, o);
+    // the window moves on by o bytes: bytes [o, o + 16) of c0..c3 then n0..n3 (which arrived meanwhile)
+    shift16(n0, n1, n2, n3, nd);
+    const uint32_t q = o >> 2, r = o & 3u;
+    const uint32_t e0 = q == 0 ? c0 : q == 1 ? c1 : q == 2 ? c2 : q == 3 ? c3 : n0;
+    const uint32_t e1 = q == 0 ? c1 : q == 1 ? c2 : q == 2 ? c3 : q == 3 ? n0 : n1;
+    const uint32_t e2 = q == 0 ? c2 : q == 1 ? c3 : q == 2 ? n0 : q == 3 ? n1 : n2;
+    const uint32_t e3 = q == 0 ? c3 : q == 1 ? n0 : q == 2 ? n1 : q == 3 ? n2 : n3;
+    const uint32_t e4 = q == 0 ? n0 : q == 1 ? n1 : q == 2 ? n2 : q == 3 ? n3 : 0u;
+    c0 = __builtin_amdgcn_alignbyte(e1, e0, r);
+    c1 = __builtin_amdgcn_alignbyte(e2, e1, r);
+    c2 = __builtin_amdgcn_alignbyte(e3, e2, r);
+    c3 = __builtin_amdgcn_alignbyte(e4, e3, r);
+    pos += o;
+    load16(lp, pos + 16, lim, n0, n1, n2, n3, nd);
+    // these 512 symbols to their place, after the load: a wait for that load then does not wait for these stores
+    wave_fence();
+    wave_copy_lds(dest + b0 * 64, stg, 512u);
+    wave_fence();
+  }
+  if (b0 < nsteps) {  // the last (at most 8) steps, within the window
+    uint32_t o = 0;
+    for (uint32_t k = 0; b0 + k < nsteps; ++k)
+      rans_step(slot, fc, stg + k * 64 + lane, (b0 + k) * 64 + lane < n, c0, c1, c2, c3, x, o);
+    pos += o;
+    wave_fence();
+    wave_copy_lds(dest + b0 * 64, stg, n - b0 * 64);
     wave_fence();
   }
   if (ballot(pos > lnl) && lane == 0) atomicOr(&err[b], E_LANE);
