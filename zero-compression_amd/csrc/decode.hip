@@ -391,6 +391,163 @@ __device__ inline void group_copy(uint8_t* o, const uint8_t* litb, uint32_t raw,
   }
 }
 
+// ---------------------------------------------------------------- the LZ replay through an LDS window
+// The last RING bytes of a block's output stay in LDS (byte y at ring[y % RING], the first 16 bytes mirrored after
+// the end so that any 16 bytes read at once). The copy chunks of 64 tokens are laid out first: each one's 16 source
+// bytes are requested from global memory together when they are literals or output older than the window, or read
+// from the window when they are recent output; then the groups are written one after another into the window, LDS
+// reads and writes only, and at the end the chunk's output goes to global memory in 16-byte stores. So a chunk of
+// tokens costs one round trip to memory instead of one per group.
+constexpr uint32_t RING = 4096;
+constexpr uint32_t RROUNDS = 2;  // copy chunks laid out at once: RROUNDS x 64
+
+__device__ inline U128 ring_read16(const uint8_t* ring, uint32_t y) {
+  U128 v;
+  __builtin_memcpy(&v, ring + (y & (RING - 1)), 16);
+  return v;
+}
+// len (1..16) bytes of v at output position y, with the wrap and the mirror kept up
+__device__ inline void ring_write(uint8_t* ring, uint32_t y, const U128& v, uint32_t len, bool on) {
+  const uint32_t i = y & (RING - 1);
+  if (on) {
+    if (len == 16u) __builtin_memcpy(ring + i, &v, 16);
+    else st_partial(ring + i, v, len);
+  }
+  const bool wrap = on && i + len > RING, mirror = on && i < 16u;
+  if (ballot(wrap || mirror)) {
+    if (wrap) st_partial(ring, shr128(v, RING - i), i + len - RING);  // (at most 15 bytes)
+    if (mirror) {
+      const uint32_t m = min(len, 16u - i);
+      if (m == 16u) __builtin_memcpy(ring + RING + i, &v, 16);
+      else st_partial(ring + RING + i, v, m);
+    }
+  }
+}
+
+// one copy chunk: where it writes (y0, len) and where its source bytes are (see chunk_count)
+struct CDesc {
+  uint32_t y0, len, s0, flags;  // flags: group | kind << 8 | pattern << 10 | phase << 11 | D << 16 (D < 16 only)
+  uint32_t k;                   // the token (index in the chunk of tokens)
+};
+enum : uint32_t { K_PRE = 0, K_RING = 1, K_SLOW = 2, K_NONE = 3 };
+
+__device__ inline CDesc describe(uint32_t cj, bool live, uint32_t k, const uint32_t* sop, const uint32_t* sL,
+                                 const uint32_t* sM, const uint32_t* sD, const uint32_t* sli, const uint32_t* scs,
+                                 const uint32_t* srg, const uint8_t* o, const uint8_t* litb, uint32_t raw, uint32_t lo,
+                                 const uint8_t*& src) {
+  CDesc d;
+  const uint32_t kk = live ? k : 0u;
+  const uint32_t opk = sop[kk], L = sL[kk], M = sM[kk], D = sD[kk], lik = sli[kk], cs = scs[kk], grp = srg[kk];
+  const uint32_t w = cj - cs;
+  const uint32_t nlit = (L + 15u) >> 4;
+  const uint32_t ms = opk + L, me = ms + M;
+  const bool litc = w < nlit;
+  const bool pat = !litc && D < 16u;
+  uint32_t rep = 0, within = litc ? w : w - nlit;
+  if (ballot(live && !litc && D >= 16u && D < M)) {
+    if (!litc && D >= 16u && D < M) divmod_small(within, (D + 15u) >> 4, rep, within);
+  }
+  const uint32_t base = litc ? opk : ms + rep * D;
+  const uint32_t y0 = base + 16u * within;
+  const uint32_t cend = litc ? ms : pat ? me : min(me, base + D);
+  const uint32_t len = live ? min(16u, cend - y0) : 0u;
+  const uint32_t s0 = litc ? y0 : pat ? ms - D : y0 - (rep + 1u) * D;
+  const uint32_t need = pat ? D : len;
+  uint32_t kind;
+  if (!live) kind = K_NONE;
+  else if (s0 >= opk) kind = K_PRE;                                    // its own literals
+  else if (s0 + need > opk) kind = K_SLOW;                             // part output, part literals
+  else if (s0 >= lo) kind = K_RING;                                    // recent output
+  else if (s0 + need > lo || s0 + 16u > raw) kind = K_SLOW;            // across the window's edge, or the block's end
+  else kind = K_PRE;                                                   // older output, in global memory
+  src = s0 >= opk ? litb + lik + (s0 - opk) : o + s0;
+  const uint32_t ph = pat ? mod_small(16u * within, D) : 0u;
+  d.y0 = y0;
+  d.len = len;
+  d.s0 = s0;
+  d.flags = grp | kind << 8 | (pat ? 1u : 0u) << 10 | ph << 11 | (pat ? D : 0u) << 16;
+  d.k = kk;
+  return d;
+}
+
+// a copy chunk byte by byte: its source bytes from the literals, the window or global memory
+__device__ inline void copy_slow(const CDesc& d, uint8_t* ring, const uint8_t* o, const uint8_t* litb, uint32_t lo,
+                                 const uint32_t* sop, const uint32_t* sli) {
+  const uint32_t opk = sop[d.k], lik = sli[d.k];
+  const bool pat = (d.flags >> 10) & 1u;
+  const uint32_t ph = (d.flags >> 11) & 15u, D = d.flags >> 16;
+  for (uint32_t i = 0; i < d.len; ++i) {
+    const uint32_t si = pat ? d.s0 + mod_small(ph + i, D) : d.s0 + i;
+    const uint8_t v8 = si >= opk ? litb[lik + (si - opk)] : si >= lo ? ring[si & (RING - 1)] : o[si];
+    const uint32_t yi = (d.y0 + i) & (RING - 1);
+    ring[yi] = v8;
+    if (yi < 16u) ring[RING + yi] = v8;
+  }
+}
+
+// the chunk of tokens' copy chunks [0, C) through the window; lo: the oldest output position the window holds
+__device__ inline void ring_copy(uint8_t* ring, const uint8_t* o, const uint8_t* litb, uint32_t raw, uint32_t lo,
+                                 uint32_t C, bool on, uint32_t ccs_lane, const uint32_t* sop, const uint32_t* sL,
+                                 const uint32_t* sM, const uint32_t* sD, const uint32_t* sli, const uint32_t* scs,
+                                 const uint32_t* srg, uint32_t* sown) {
+  const uint32_t lane = lane_id();
+  for (uint32_t base = 0; base < C; base += 64 * RROUNDS) {
+    CDesc d[RROUNDS];
+    U128 pre[RROUNDS];
+#pragma unroll
+    for (uint32_t r = 0; r < RROUNDS; ++r) {
+      const uint32_t c0 = base + 64 * r;
+      sown[lane] = 0;
+      wave_fence();
+      const uint32_t jk = ccs_lane <= c0 ? 0u : ccs_lane - c0;
+      if (on && jk < 64) atomicMax(&sown[jk], lane + 1);
+      wave_fence();
+      const uint32_t k = min(scan_max(sown[lane]), 64u) - 1u;
+      wave_fence();
+      const uint32_t cj = c0 + lane;
+      const uint8_t* src;
+      d[r] = describe(cj, cj < C, k, sop, sL, sM, sD, sli, scs, srg, o, litb, raw, lo, src);
+      pre[r] = ld128(((d[r].flags >> 8) & 3u) == K_PRE ? src : litb);
+    }
+    // in chunk order, which is group order: round by round, and inside a round group by group (a chunk reads only
+    // what earlier groups wrote, and a group's chunks in later rounds come after its chunks in this one)
+#pragma unroll
+    for (uint32_t r = 0; r < RROUNDS; ++r) {
+      const uint32_t kind = (d[r].flags >> 8) & 3u;
+      const uint32_t grp = d[r].flags & 255u;
+      const bool live = kind != K_NONE;
+      if (!ballot(live)) break;
+      const uint32_t qa = wave_min(live ? grp : 255u), qb = wave_max(live ? grp : 0u);
+      for (uint32_t q = qa; q <= qb; ++q) {
+        const bool act = live && grp == q;
+        if (!ballot(act)) continue;
+        U128 v = pre[r];
+        if (ballot(act && kind == K_RING)) {
+          if (act && kind == K_RING) v = ring_read16(ring, d[r].s0);
+        }
+        const bool pat = (d[r].flags >> 10) & 1u;
+        if (ballot(act && pat && kind != K_SLOW)) {
+          if (act && pat && kind != K_SLOW) v = pattern16(v, (d[r].flags >> 11) & 15u, d[r].flags >> 16);
+        }
+        ring_write(ring, d[r].y0, v, d[r].len, act && kind != K_SLOW);
+        if (ballot(act && kind == K_SLOW)) {
+          if (act && kind == K_SLOW) copy_slow(d[r], ring, o, litb, lo, sop, sli);
+        }
+        wave_fence();  // (this group's bytes are in the window before the next group reads them)
+      }
+    }
+  }
+}
+
+// output [a, e) of the block, now in the window, to global memory
+__device__ inline void ring_flush(const uint8_t* ring, uint8_t* o, uint32_t a, uint32_t e) {
+  const uint32_t lane = lane_id();
+  const uint32_t A = min((a + 15u) & ~15u, e), E = max(A, e & ~15u);
+  if (a + lane < A) o[a + lane] = ring[(a + lane) & (RING - 1)];
+  for (uint32_t y = A + 16u * lane; y < E; y += 64u * 16u) st128(o + y, ring_read16(ring, y));
+  if (E + lane < e) o[E + lane] = ring[(E + lane) & (RING - 1)];
+}
+
 // Tokens go 64 at a time: a chunk may hold the end of one group and several whole groups. The fields, positions
 // and offsets of the whole chunk are computed together (scans by DPP), with the next chunk's bytes already
 // requested; then the chunk's groups are written one after another (a match reads output from before its group
@@ -403,7 +560,8 @@ __global__ __launch_bounds__(64) void k_dec_lz(const Block* __restrict__ blocks,
                                                uint8_t* out, uint32_t* __restrict__ err) {
   __shared__ uint32_t sflag[64];  // token k of the chunk starts a group
   __shared__ uint32_t sgs[64];    // output start of the chunk's k-th group
-  __shared__ uint32_t sop[64], sL[64], sM[64], sD[64], sli[64], scs[64], sown[64];
+  __shared__ uint32_t sop[64], sL[64], sM[64], sD[64], sli[64], scs[64], srg[64], sown[64];
+  __shared__ uint8_t ring[RING + 16];  // the block's most recent output
   const Geo G = geo_of(blog);
   const uint32_t b = blockIdx.x;
   const uint32_t lane = threadIdx.x;
@@ -432,6 +590,7 @@ __global__ __launch_bounds__(64) void k_dec_lz(const Block* __restrict__ blocks,
   uint32_t t = 0, op = 0, li = 0, ei = 0, bp = 0;
   uint32_t g = 0, grem = 0, gs_open = 0;  // groups started so far; tokens left in the open group; its output start
   uint32_t r0 = 1, r1 = 4, r2 = 8, l0 = 1, l1 = 4, l2 = 8;
+  uint32_t ring_lo = 0;  // output before this is not in the window (it went to global memory another way)
   uint32_t fail = 0;
   uint32_t tv_n = lane < nt ? tokb[lane] : 0u, sv_n = lane < nt ? symb[lane] : 0u;
   uint32_t gw_n = lane < ng ? grpb[lane] : 0u;
@@ -524,7 +683,7 @@ __global__ __launch_bounds__(64) void k_dec_lz(const Block* __restrict__ blocks,
     if (ballot(bad || gbad)) { fail = ballot(bad) ? E_TOKEN : E_GROUP; break; }
     const uint32_t cnt = on ? chunk_count(L, M, D) : 0u;
     const uint32_t cinc = scan_add(cnt), ccs = cinc - cnt;
-    sop[lane] = opk; sL[lane] = L; sM[lane] = M; sD[lane] = D; sli[lane] = lik; scs[lane] = ccs;
+    sop[lane] = opk; sL[lane] = L; sM[lane] = M; sD[lane] = D; sli[lane] = lik; scs[lane] = ccs; srg[lane] = rg;
     wave_fence();
     if (grem2 && nstart) gs_open = readlane(opk, readlane(gbeg, nstart - 1));  // (the open group's first token)
     const uint32_t op_end = op + readlane(itot, 63);
@@ -535,13 +694,22 @@ __global__ __launch_bounds__(64) void k_dec_lz(const Block* __restrict__ blocks,
     t += c;
     g = g2;
     grem = grem2;
-    for (uint32_t q = 0; q < ngc; ++q) {
-      const uint64_t qm = ballot(on && rg == q);
-      if (!qm) continue;
-      const uint32_t ka = (uint32_t)__builtin_ctzll(qm), kb = 64u - (uint32_t)__builtin_clzll(qm);
-      group_copy(o, litb, raw, on && rg == q, ccs, readlane(ccs, ka), readlane(cinc, kb - 1), sop, sL, sM, sD, sli,
-                 scs, sown);
-      wave_fence();  // this group's output is read by the next
+    const uint32_t op0 = op_end - readlane(itot, 63);  // (the chunk's output: [op0, op_end))
+    if (op_end - op0 <= RING - 16u) {
+      const uint32_t lo = max(op_end > RING ? op_end - RING : 0u, ring_lo);
+      ring_copy(ring, o, litb, raw, lo, readlane(cinc, 63), on, ccs, sop, sL, sM, sD, sli, scs, srg, sown);
+      wave_fence();
+      ring_flush(ring, o, op0, op_end);
+    } else {  // a long chunk: group by group through global memory, and the window starts again after it
+      for (uint32_t q = 0; q < ngc; ++q) {
+        const uint64_t qm = ballot(on && rg == q);
+        if (!qm) continue;
+        const uint32_t ka = (uint32_t)__builtin_ctzll(qm), kb = 64u - (uint32_t)__builtin_clzll(qm);
+        group_copy(o, litb, raw, on && rg == q, ccs, readlane(ccs, ka), readlane(cinc, kb - 1), sop, sL, sM, sD, sli,
+                   scs, sown);
+        wave_fence();  // this group's output is read by the next
+      }
+      ring_lo = op_end;
     }
     wave_fence();
   }
