@@ -50,6 +50,14 @@ __device__ inline uint32_t prefix_len(const uint8_t* inb, uint32_t a, uint32_t b
 
 __device__ inline uint32_t bitlen(uint32_t d) { return 31u - (uint32_t)__builtin_clz(d); }
 
+// The dispatcher hands workgroup i to XCD i % 8. A position-parallel kernel reads and writes one block's arrays
+// (1 MiB each at 2^18) from a thousand workgroups; renumbered so that each XCD takes a contiguous run of them, a
+// block's lines are filled in one XCD's L2 instead of partly in all eight.
+__device__ inline uint32_t xcd_swizzle(uint32_t i, uint32_t G) {
+  const uint32_t per = G / 8u, rem = G % 8u, x = i % 8u, j = i / 8u;
+  return x * per + min(x, rem) + j;
+}
+
 // level 1..9 -> (NCANDL, LENCAP): neighbours compared and bytes compared before the extension (src/gpu_codec.h:66-70)
 __host__ __device__ inline void level_params(uint32_t level, uint32_t& ncand, uint32_t& lencap) {
   ncand = level <= 2 ? 2u : level <= 6 ? 4u : 8u;
@@ -130,7 +138,7 @@ __global__ __launch_bounds__(256) void k_enc_sort_s(const Block* __restrict__ bl
                                                     uint32_t hbits, uint32_t* __restrict__ srt,
                                                     const uint32_t* __restrict__ bstart, SortQueues q) {
   const uint32_t HB = 1u << hbits;
-  const uint64_t gid = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  const uint64_t gid = (uint64_t)xcd_swizzle(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   const uint32_t b = (uint32_t)(gid / HB), h = (uint32_t)(gid % HB);
   const bool valid = b < nb;
   uint32_t lo = 0, k = 0;
@@ -288,8 +296,9 @@ __global__ __launch_bounds__(256) void k_enc_match(const Block* __restrict__ blo
                                                    const uint32_t* __restrict__ srt, const uint32_t* __restrict__ bstart,
                                                    uint32_t* __restrict__ match) {
   const uint32_t B = 1u << blog;
-  const uint32_t b = blockIdx.x >> (blog - 8);
-  const uint32_t i = ((blockIdx.x << 8) + threadIdx.x) & (B - 1u);
+  const uint32_t wg = xcd_swizzle(blockIdx.x, gridDim.x);
+  const uint32_t b = wg >> (blog - 8);
+  const uint32_t i = ((wg << 8) + threadIdx.x) & (B - 1u);
   const Block blk = blocks[b];
   const uint32_t n = blk.raw_len;
   if (i >= n) return;
@@ -330,8 +339,9 @@ __global__ __launch_bounds__(256) void k_enc_match(const Block* __restrict__ blo
 __global__ __launch_bounds__(256) void k_enc_lazy(const Block* __restrict__ blocks, uint32_t blog,
                                                   const uint32_t* __restrict__ match, uint32_t* __restrict__ lazy) {
   const uint32_t B = 1u << blog;
-  const uint32_t b = blockIdx.x >> (blog - 8);
-  const uint32_t p = ((blockIdx.x << 8) + threadIdx.x) & (B - 1u);
+  const uint32_t wg = xcd_swizzle(blockIdx.x, gridDim.x);
+  const uint32_t b = wg >> (blog - 8);
+  const uint32_t p = ((wg << 8) + threadIdx.x) & (B - 1u);
   const uint32_t n = blocks[b].raw_len;
   if (p >= n) return;
   const size_t at = ((size_t)b << blog) + p;
