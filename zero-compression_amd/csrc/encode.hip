@@ -680,22 +680,53 @@ __global__ __launch_bounds__(64) void k_enc_rans(uint32_t blog, TokBufs T, RansB
     c += f[k];
   }
   __syncthreads();
-  // lane j encodes symbols j, j+64, ... from the last one back into its scratch, backwards (src/codec.h:112-126)
+  // lane j encodes symbols j, j+64, ... from the last one back into its scratch, backwards (src/codec.h:112-126).
+  // Steps go 4 at a time: the 256 symbols of 4 steps arrive in LDS from one coalesced load made a group earlier,
+  // and the (at most 8) bytes the lane emits in them leave in one 8-byte store ending where its bytes stand (the
+  // bytes below them in that store are garbage that the next group's store covers). Every group makes exactly one
+  // load and one store, so a wait for the load never has to wait for the stores after it.
+  __shared__ uint32_t stage[64];
   const uint32_t LC = lane_cap(G, s);
   uint8_t* end = R.lanes + (size_t)b * lanes_per_block(G) + lane_off(G, s) + (size_t)(lane + 1) * LC;
   uint8_t* ptr = end;
   uint32_t x = 1u << 23;
-  if (lane < n) {
-    for (int64_t i = (int64_t)lane + (int64_t)((n - 1 - lane) / 64u) * 64; i >= 0; i -= 64) {
-      const uint32_t e = fc[sy[i]];
+  const uint32_t NT = (n + 63u) / 64u;                 // steps: symbol lane + 64 t at step t, t from NT-1 down
+  int32_t j = (int32_t)((NT - 1u) >> 2);               // groups of steps [4j, 4j + 3]
+  uint32_t nxt;
+  __builtin_memcpy(&nxt, sy + 256u * (uint32_t)j + 4u * lane, 4);
+  for (; j >= 0; --j) {
+    wave_fence();
+    stage[lane] = nxt;
+    wave_fence();
+    if (j > 0) __builtin_memcpy(&nxt, sy + 256u * (uint32_t)(j - 1) + 4u * lane, 4);
+    uint64_t acc = 0;
+    uint32_t k = 0;  // bytes emitted in this group, the first at the top of acc
+    const uint8_t* st8 = (const uint8_t*)stage;
+#pragma unroll
+    for (int q = 3; q >= 0; --q) {
+      const uint32_t t = 4u * (uint32_t)j + (uint32_t)q;
+      const bool valid = t < NT && lane + 64u * t < n;
+      const uint32_t sym = st8[64u * (uint32_t)q + lane];
+      const uint32_t e = fc[sym];
       const uint32_t fq = e & 0xFFFFu;
       const uint32_t xmax = fq << 19;
-      while (x >= xmax) { *--ptr = (uint8_t)x; x >>= 8; }
+      // at most two bytes leave (x < 2^31, and x < 2^23 < ... after one; fq = 1 may take a second)
+      const bool e1 = valid && x >= xmax;
+      const uint32_t x1 = e1 ? x >> 8 : x;
+      const bool e2 = e1 && x1 >= xmax;
+      const uint32_t x2 = e2 ? x1 >> 8 : x1;
+      acc |= e1 ? (uint64_t)(x & 255u) << (8u * (7u - k)) : 0ull;
+      k += e1 ? 1u : 0u;
+      acc |= e2 ? (uint64_t)(x1 & 255u) << (8u * (7u - k)) : 0ull;
+      k += e2 ? 1u : 0u;
       // exact x / fq for x < 2^31: floor(x * ceil(2^(31+s)/fq) / 2^(31+s)) with 2^s >= fq
       const uint32_t sh = fq > 1 ? 32u - (uint32_t)__builtin_clz(fq - 1) : 0u;
-      const uint32_t qt = (uint32_t)(((uint64_t)x * rcp[sy[i]]) >> (31 + sh));
-      x = (qt << 12) + (x - qt * fq) + (e >> 16);
+      const uint32_t qt = (uint32_t)(((uint64_t)x2 * rcp[sym]) >> (31 + sh));
+      const uint32_t xn = (qt << 12) + (x2 - qt * fq) + (e >> 16);
+      x = valid ? xn : x;
     }
+    __builtin_memcpy(ptr - 8, &acc, 8);
+    ptr -= k;
   }
   ptr -= 4;
   ptr[0] = (uint8_t)x; ptr[1] = (uint8_t)(x >> 8); ptr[2] = (uint8_t)(x >> 16); ptr[3] = (uint8_t)(x >> 24);
