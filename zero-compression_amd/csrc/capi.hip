@@ -227,6 +227,8 @@ int zgpu_open(int ordinal, zgpu_t** out) {
   }
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_enc_hash), hipFuncAttributeMaxDynamicSharedMemorySize,
                             160 * 1024);
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_enc_place), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            160 * 1024);
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_enc_parse), hipFuncAttributeMaxDynamicSharedMemorySize,
                             160 * 1024);
   if (const char* pe = getenv("ZERO_PROFILE")) g->profile = atoi(pe) != 0;
@@ -361,7 +363,10 @@ static int encode_run(zgpu_t* g, const uint8_t* d_in, const Block* blocks, uint3
   const uint32_t posgroups = nb << (blog - 8);
   if (prof) stage_begin(g, st);
   k_fill<<<1, 64, 0, st>>>((uint8_t*)q.counts, 0, 16);
-  k_enc_hash<<<nb, 1024, (HB + 1024) * 4, st>>>(blocks, d_in, blog, S.hbits, S.keybits, srt, bstart);
+  const uint32_t pbits = pbits_of(blog, S.hbits), NP = 1u << pbits;
+  k_enc_hash<<<nb, 1024, (HB + 1024 + 18 * NP) * 4, st>>>(blocks, d_in, blog, S.hbits, pbits, mt, bstart);
+  k_enc_place<<<nb << pbits, 1024, ((1u << (S.hbits - pbits)) + PLACE_CAP) * 4, st>>>(blocks, d_in, nb, blog, S.hbits,
+                                                                                     S.keybits, pbits, mt, bstart, srt);
   if (prof) stage(g, st, "hash");
   k_enc_sort_s<<<(unsigned)(((size_t)nb * HB + 255) / 256), 256, 0, st>>>(blocks, nb, blog, S.hbits, srt, bstart, q);
   k_enc_sort_m<<<std::min<uint32_t>(2048, L.qcap), 256, 0, st>>>(blog, S.hbits, srt, bstart, q);

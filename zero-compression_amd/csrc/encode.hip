@@ -64,15 +64,26 @@ __host__ __device__ inline void level_params(uint32_t level, uint32_t& ncand, ui
   lencap = level <= 6 ? 16u : 32u;
 }
 
-// ---------------------------------------------------------------- 1. hash, count, scatter (one workgroup per block)
+// ---------------------------------------------------------------- 1. hash, count, scatter
 // The key of a position is the top `keybits` bits of its hash; buckets are its top `hbits` bits. srt[b*B + slot] =
 // (the key's other keybits - hbits bits) << blog | position, grouped by bucket; bstart[b*(HB+1) + h] = the first slot
 // of bucket h. Once each bucket is sorted, positions are in (key, position) order whatever hbits is, so hbits (and
 // the slab size 2^blog, any size that holds the block) changes the work, not the candidates.
+//
+// Two kernels, so that no write lands at a random place in a block's 4-byte-per-position arrays (such writes cost
+// whole lines of HBM traffic each): k_enc_hash (one workgroup per block) counts the buckets, writes bstart, and puts
+// the positions in partition order into tmp, a partition being the buckets that share the top pbits hash bits
+// (about 8192 positions); k_enc_place (one workgroup per partition) puts a partition's entries in bucket order in LDS
+// and writes them out in order.
+constexpr uint32_t PLACE_CAP = 15360;  // entries of a partition placed in LDS (larger ones go straight to srt)
+__host__ __device__ inline uint32_t pbits_of(uint32_t blog, uint32_t hbits) {
+  const uint32_t want = blog > 13u ? blog - 13u : 0u;
+  return want < hbits ? want : hbits;
+}
 __global__ __launch_bounds__(1024) void k_enc_hash(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
-                                                   uint32_t blog, uint32_t hbits, uint32_t keybits,
-                                                   uint32_t* __restrict__ srt, uint32_t* __restrict__ bstart) {
-  extern __shared__ uint32_t cnt[];  // HB counters, then HB/1024 partial sums
+                                                   uint32_t blog, uint32_t hbits, uint32_t pbits,
+                                                   uint32_t* __restrict__ tmp, uint32_t* __restrict__ bstart) {
+  extern __shared__ uint32_t cnt[];  // HB counters, HB/1024 partial sums, then the partition ranking's tables
   const uint32_t b = blockIdx.x, tid = threadIdx.x;
   const uint32_t HBITS = hbits, HB = 1u << HBITS;
   const Block blk = blocks[b];
@@ -114,13 +125,79 @@ __global__ __launch_bounds__(1024) void k_enc_hash(const Block* __restrict__ blo
   }
   if (tid == 0) bs[HB] = count;
   __syncthreads();
-  uint32_t* sb = srt + ((size_t)b << blog);
-  const uint32_t tagmask = (1u << (keybits - HBITS)) - 1u;
-  for (uint32_t p = tid; p < count; p += 1024) {
-    const uint32_t h = hash5(inb, n, p);
-    const uint32_t slot = atomicAdd(&cnt[h >> (32 - HBITS)], 1u);
-    sb[slot] = ((h >> (32 - keybits)) & tagmask) << blog | p;
+  // the positions in partition order, stable, 1024 at a time: ranks inside a wavefront from ballots over the
+  // partition's bits, then across the 16 wavefronts through LDS
+  const uint32_t NP = 1u << pbits;
+  uint32_t* wc = cnt + HB + 1024;  // [16][NP]
+  uint32_t* pcur = wc + 16 * NP;   // [NP] next slot of each partition
+  uint32_t* tt = pcur + NP;        // [NP] this round's count of each partition
+  for (uint32_t i = tid; i < 16 * NP; i += 1024) wc[i] = 0;
+  if (tid < NP) pcur[tid] = cnt[tid << (HBITS - pbits)];
+  __syncthreads();
+  uint32_t* tb = tmp + ((size_t)b << blog);
+  const uint32_t wv = tid >> 6;
+  for (uint32_t t0 = 0; t0 < count; t0 += 1024) {
+    const uint32_t p = t0 + tid;
+    const bool valid = p < count;
+    const uint32_t d = valid && pbits ? hash5(inb, n, p) >> (32 - pbits) : 0u;
+    uint64_t m = ballot(valid);
+    for (uint32_t bit = 0; bit < pbits; ++bit) {
+      const uint64_t bb = ballot((d >> bit) & 1u);
+      m &= ((d >> bit) & 1u) ? bb : ~bb;
+    }
+    const uint32_t rank = popc64(m & lanemask_lt());
+    if (valid && rank == 0) wc[wv * NP + d] = popc64(m);
+    __syncthreads();
+    if (tid < NP) {
+      uint32_t run = 0;
+      for (uint32_t x = 0; x < 16; ++x) { const uint32_t c = wc[x * NP + tid]; wc[x * NP + tid] = run; run += c; }
+      tt[tid] = run;
+    }
+    __syncthreads();
+    if (valid) tb[pcur[d] + wc[wv * NP + d] + rank] = p;
+    __syncthreads();
+    if (tid < NP) {
+      pcur[tid] += tt[tid];
+      for (uint32_t x = 0; x < 16; ++x) wc[x * NP + tid] = 0;
+    }
+    __syncthreads();
   }
+}
+
+// one workgroup per (block, partition): the partition's positions (in position order, from tmp) to their buckets
+__global__ __launch_bounds__(1024) void k_enc_place(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
+                                                    uint32_t nb, uint32_t blog, uint32_t hbits, uint32_t keybits,
+                                                    uint32_t pbits, const uint32_t* __restrict__ tmp,
+                                                    const uint32_t* __restrict__ bstart, uint32_t* __restrict__ srt) {
+  extern __shared__ uint32_t sm[];  // bucket cursors (2^(hbits - pbits)), then the partition's entries
+  const uint32_t wg = xcd_swizzle(blockIdx.x, gridDim.x);
+  const uint32_t b = wg >> pbits, P = wg & ((1u << pbits) - 1u), tid = threadIdx.x;
+  if (b >= nb) return;
+  const uint32_t HB = 1u << hbits, pb = hbits - pbits, NB = 1u << pb;
+  const Block blk = blocks[b];
+  const uint8_t* inb = in + blk.src;
+  const uint32_t n = blk.raw_len;
+  const uint32_t* bs = bstart + (size_t)b * (HB + 1);
+  const uint32_t ps = bs[P << pb], pe = bs[(P + 1) << pb], m = pe - ps;
+  uint32_t* lc = sm;
+  uint32_t* buf = sm + NB;
+  for (uint32_t i = tid; i < NB; i += 1024) lc[i] = bs[(P << pb) + i] - ps;
+  __syncthreads();
+  const bool in_lds = m <= PLACE_CAP;
+  const uint32_t* tb = tmp + ((size_t)b << blog) + ps;
+  uint32_t* sb = srt + ((size_t)b << blog) + ps;
+  const uint32_t tagmask = (1u << (keybits - hbits)) - 1u;
+  for (uint32_t i = tid; i < m; i += 1024) {
+    const uint32_t p = tb[i];
+    const uint32_t h = hash5(inb, n, p);
+    const uint32_t slot = atomicAdd(&lc[(h >> (32 - hbits)) & (NB - 1u)], 1u);
+    const uint32_t e = ((h >> (32 - keybits)) & tagmask) << blog | p;
+    if (in_lds) buf[slot] = e;
+    else sb[slot] = e;
+  }
+  __syncthreads();
+  if (in_lds)
+    for (uint32_t i = tid; i < m; i += 1024) sb[i] = buf[i];
 }
 
 // ---------------------------------------------------------------- 2. bucket sorts (the values are distinct)

@@ -47,7 +47,9 @@ size_t parse_lds_bytes(const EncShape& S) {
   return (B / 32 + 256 + B / S.walkers) * 4;
 }
 bool shape_fits(const EncShape& S) {
-  return shape_ok(S) && parse_lds_bytes(S) <= 160 * 1024 && (((size_t)1 << S.hbits) + 1024) * 4 <= 160 * 1024;
+  const size_t np = (size_t)1 << pbits_of(S.blog, S.hbits);
+  return shape_ok(S) && parse_lds_bytes(S) <= 160 * 1024 && (((size_t)1 << S.hbits) + 1024 + 18 * np) * 4 <= 160 * 1024 &&
+         (((size_t)1 << (S.hbits - pbits_of(S.blog, S.hbits))) + PLACE_CAP) * 4 <= 160 * 1024;
 }
 
 }  // namespace
