@@ -372,7 +372,7 @@ static int encode_run(zgpu_t* g, const uint8_t* d_in, const Block* blocks, uint3
   k_enc_sort_m<<<std::min<uint32_t>(2048, L.qcap), 256, 0, st>>>(blog, S.hbits, srt, bstart, q);
   k_enc_sort_l<<<std::min<uint32_t>(1024, L.qcap), 1024, 0, st>>>(blog, S.hbits, srt, mt, bstart, q);
   if (prof) stage(g, st, "sort");
-  k_enc_match<<<posgroups, 256, 0, st>>>(blocks, d_in, blog, S.hbits, ncand, lencap, srt, bstart, mt);
+  k_enc_match<<<posgroups, 256, 0, st>>>(blocks, d_in, blog, S.keybits, ncand, lencap, srt, mt);
   if (prof) stage(g, st, "match");
   k_enc_lazy<<<posgroups, 256, 0, st>>>(blocks, blog, mt, srt);
   if (prof) stage(g, st, "lazy");

@@ -368,48 +368,128 @@ __global__ __launch_bounds__(1024) void k_enc_sort_l(uint32_t blog, uint32_t hbi
 }
 
 // ---------------------------------------------------------------- 3. match finder (one thread per sorted slot)
-__global__ __launch_bounds__(256) void k_enc_match(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
-                                                   uint32_t blog, uint32_t hbits, uint32_t ncand, uint32_t lencap,
-                                                   const uint32_t* __restrict__ srt, const uint32_t* __restrict__ bstart,
-                                                   uint32_t* __restrict__ match) {
+// The best match of a position is among its run check (distance 1) and the ncand slots just before its own in the
+// sorted order that hold the same key (kernels/zero.cl:172-236): the longest, the nearest on a tie. A wavefront takes
+// 64 consecutive slots of one block. Each lane reads the first W bytes of its position once (W = 16 for a length cap
+// of 16, 32 for one of 32; bytes past the block read as 0) and lanes 0..7 also those of the 8 slots before the
+// wavefront's first; windows, positions and keys go to LDS, and each candidate is compared there, window against
+// window. The key of a candidate is its hash recomputed from its window, so that a different key (another bucket, or
+// the same bucket and another tag) ends the candidates as the sorted order's run of one key ends.
+constexpr uint32_t MX = 8;             // most candidates of any level
+constexpr uint32_t NOPOS = 0xFFFFFFFFu;  // an LDS entry with no slot behind it
+
+template <int NW>
+__device__ inline void load_window(const uint8_t* inb, uint32_t n, uint32_t p, uint32_t (&w)[NW]) {
+  if (p + 4u * NW <= n) {
+#pragma unroll
+    for (int q = 0; q < NW; q += 4) {
+      uint32_t v[4];
+      __builtin_memcpy(v, inb + p + 4 * q, 16);
+      w[q] = v[0]; w[q + 1] = v[1]; w[q + 2] = v[2]; w[q + 3] = v[3];
+    }
+  } else {  // the last positions of a block
+#pragma unroll
+    for (int q = 0; q < NW; ++q) w[q] = 0;
+#pragma unroll
+    for (int t = 0; t < 4 * NW; ++t)
+      if (p + t < n) w[t >> 2] |= (uint32_t)inb[p + t] << (8 * (t & 3));
+  }
+}
+
+// the key of the position whose window is w (hash5 of its first 5 bytes, the fifth read as 0 past the block)
+template <int NW>
+__device__ inline uint32_t window_key(const uint32_t (&w)[NW], uint32_t keybits) {
+  const uint32_t x = w[1] & 0xFFu;
+  return ((w[0] * 2654435761u) ^ (x * 0x9E3779B1u + (x >> 3))) >> (32 - keybits);
+}
+
+// the number of leading bytes where a and b agree (4 * NW when all do)
+template <int NW>
+__device__ inline uint32_t common_prefix(const uint32_t (&a)[NW], const uint32_t (&b)[NW]) {
+  uint32_t l = 4u * NW;
+#pragma unroll
+  for (int q = NW - 1; q >= 0; --q) {
+    const uint32_t x = a[q] ^ b[q];
+    if (x) l = 4u * q + ((uint32_t)__builtin_ctz(x) >> 3);
+  }
+  return l;
+}
+
+template <int NW>
+__device__ inline void match_slots(const uint8_t* __restrict__ inb, uint32_t n, uint32_t count, uint32_t blog,
+                                   uint32_t keybits, uint32_t ncand, uint32_t lencap, const uint32_t* __restrict__ sb,
+                                   uint32_t i0, uint32_t* __restrict__ mb, uint32_t* sh) {
+  const uint32_t lane = threadIdx.x & 63u, i = i0 + lane;
   const uint32_t B = 1u << blog;
-  const uint32_t wg = xcd_swizzle(blockIdx.x, gridDim.x);
-  const uint32_t b = wg >> (blog - 8);
-  const uint32_t i = ((wg << 8) + threadIdx.x) & (B - 1u);
-  const Block blk = blocks[b];
-  const uint32_t n = blk.raw_len;
-  if (i >= n) return;
-  if (blk.level) level_params(blk.level, ncand, lencap);
-  const uint32_t count = n >= 4 ? n - 3 : 0;
-  uint32_t* mb = match + ((size_t)b << blog);
-  if (i >= count) { mb[i] = 0; return; }  // (the last 3 positions are never in srt: exactly these slots)
-  const uint8_t* inb = in + blk.src;
-  const uint32_t* sb = srt + ((size_t)b << blog);
-  const uint32_t HBITS = hbits, HB = 1u << HBITS;
-  const uint32_t e = sb[i];
-  const uint32_t p = e & (B - 1u), tag = e >> blog;
-  if (p >= count) return;  // (never with a correct sort; keeps every read inside the block)
-  const uint32_t bucket = hash5(inb, n, p) >> (32 - HBITS);
-  const uint32_t bfirst = bstart[(size_t)b * (HB + 1) + bucket];
-  const uint32_t pw = ld32(inb + p);
+  uint32_t* lpos = sh;                // [MX + 64]: entry MX + l is lane l's slot, entry x < MX slot i0 - MX + x
+  uint32_t* lkey = sh + MX + 64;
+  uint32_t* lwin = sh + 2 * (MX + 64);  // [MX + 64][NW]
+  const bool xact = lane < MX && i0 + lane >= MX;
+  uint32_t p = i < count ? sb[i] & (B - 1u) : 0u;
+  const uint32_t xp = xact ? sb[i0 + lane - MX] & (B - 1u) : 0u;
+  const bool act = i < count && p < count;  // (p < count always with a correct sort; keeps every read in the block)
+  if (!act) p = 0;
+  uint32_t w[NW], xw[NW];
+  load_window<NW>(inb, n, p, w);
+  const bool run = act && p > 0 && w[0] == (uint32_t)inb[p - (p > 0)] * 0x01010101u;
+  if (lane < MX) load_window<NW>(inb, n, xp < count ? xp : 0u, xw);
+  const uint32_t key = window_key<NW>(w, keybits);
+  lpos[MX + lane] = act ? p : NOPOS;
+  lkey[MX + lane] = key;
+#pragma unroll
+  for (int q = 0; q < NW; ++q) lwin[(MX + lane) * NW + q] = w[q];
+  if (lane < MX) {
+    lpos[lane] = xact ? xp : NOPOS;
+    lkey[lane] = window_key<NW>(xw, keybits);
+#pragma unroll
+    for (int q = 0; q < NW; ++q) lwin[lane * NW + q] = xw[q];
+  }
+  wave_fence();
   const uint32_t maxlen = min(lencap, n - p);
   uint32_t best = 0, bestd = 0;
-  if (p > 0 && ld32(inb + p - 1) == pw) {  // a run of one byte value (kernels/zero.cl:219-222)
-    best = 4 + prefix_len(inb, p + 3, p + 4, maxlen - 4);
+  if (run) {  // a run of one byte value (kernels/zero.cl:219-222): 5 equal bytes from p - 1
+    uint32_t r[NW];
+#pragma unroll
+    for (int q = 0; q < NW; ++q) r[q] = w[0];
+    best = min(common_prefix<NW>(w, r), maxlen);
     bestd = 1;
   }
-  for (uint32_t k = 1; k <= ncand && best < maxlen; ++k) {
-    if (i < bfirst + k) break;
-    const uint32_t c = sb[i - k];
-    if ((c >> blog) != tag) break;  // (sorted by key: no earlier slot has this key)
-    const uint32_t j = c & (B - 1u);
-    if (j >= p || ld32(inb + j) != pw) continue;
-    if (best >= 4 && inb[j + best] != inb[p + best]) continue;
-    const uint32_t l = 4 + prefix_len(inb, j + 4, p + 4, maxlen - 4);
-    const uint32_t d = p - j;
-    if (l > best || (l == best && d < bestd)) { best = l; bestd = d; }
+  bool go = act;
+  for (uint32_t k = 1; k <= ncand; ++k) {
+    const uint32_t idx = MX + lane - k;
+    const uint32_t cp = lpos[idx], ck = lkey[idx];
+    go = go && cp != NOPOS && ck == key;  // (sorted by key: no earlier slot has this key)
+    if (go && best < maxlen) {
+      uint32_t c[NW];
+#pragma unroll
+      for (int q = 0; q < NW; ++q) c[q] = lwin[idx * NW + q];
+      const uint32_t l = min(common_prefix<NW>(w, c), maxlen);
+      if (l >= 4 && l > best) { best = l; bestd = p - cp; }
+    }
   }
-  mb[p] = best >= 4 ? (best << 20) | (bestd - 1) : 0u;
+  if (act) mb[p] = best >= 4 ? (best << 20) | (bestd - 1) : 0u;
+  else if (i >= count && i < n) mb[i] = 0;  // (the last 3 positions are never in srt: exactly these slots)
+}
+
+__global__ __launch_bounds__(256) void k_enc_match(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
+                                                   uint32_t blog, uint32_t keybits, uint32_t ncand, uint32_t lencap,
+                                                   const uint32_t* __restrict__ srt, uint32_t* __restrict__ match) {
+  __shared__ uint32_t sh[4][(MX + 64) * 10];
+  const uint32_t wg = xcd_swizzle(blockIdx.x, gridDim.x);
+  const uint32_t b = wg >> (blog - 8);
+  const uint32_t i0 = ((wg << 8) + (threadIdx.x & ~63u)) & ((1u << blog) - 1u);
+  const Block blk = blocks[b];
+  const uint32_t n = blk.raw_len;
+  if (i0 >= n) return;  // (uniform over the wavefront)
+  if (blk.level) level_params(blk.level, ncand, lencap);
+  const uint32_t count = n >= 4 ? n - 3 : 0;
+  const uint8_t* inb = in + blk.src;
+  const uint32_t* sb = srt + ((size_t)b << blog);
+  uint32_t* mb = match + ((size_t)b << blog);
+  if (lencap <= 16)
+    match_slots<4>(inb, n, count, blog, keybits, min(ncand, MX), lencap, sb, i0, mb, sh[threadIdx.x >> 6]);
+  else
+    match_slots<8>(inb, n, count, blog, keybits, min(ncand, MX), min(lencap, 32u), sb, i0, mb, sh[threadIdx.x >> 6]);
 }
 
 // ---------------------------------------------------------------- 4. lazy matching (kernels/zero.cl:239-255)
