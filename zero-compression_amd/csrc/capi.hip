@@ -364,7 +364,7 @@ static int encode_run(zgpu_t* g, const uint8_t* d_in, const Block* blocks, uint3
   if (prof) stage_begin(g, st);
   k_fill<<<1, 64, 0, st>>>((uint8_t*)q.counts, 0, 16);
   const uint32_t pbits = pbits_of(blog, S.hbits), NP = 1u << pbits;
-  k_enc_hash<<<nb, 1024, (HB + 1024 + 18 * NP) * 4, st>>>(blocks, d_in, blog, S.hbits, pbits, mt, bstart);
+  k_enc_hash<<<nb, 1024, (HB + 16 + 17 * NP) * 4, st>>>(blocks, d_in, blog, S.hbits, pbits, mt, bstart);
   k_enc_place<<<nb << pbits, 1024, ((1u << (S.hbits - pbits)) + PLACE_CAP) * 4, st>>>(blocks, d_in, nb, blog, S.hbits,
                                                                                      S.keybits, pbits, mt, bstart, srt);
   if (prof) stage(g, st, "hash");

@@ -64,6 +64,23 @@ __host__ __device__ inline void level_params(uint32_t level, uint32_t& ncand, ui
   lencap = level <= 6 ? 16u : 32u;
 }
 
+// the 8 bytes from q (bytes past the block read as 0), one load at a clamped address; n >= 8
+__device__ inline uint64_t ld8z(const uint8_t* inb, uint32_t n, uint32_t q) {
+  const uint32_t qq = min(q, n), a = min(qq, n - 8u), sh = 8u * (qq - a);
+  const uint64_t x = ld64(inb + a);
+  return sh >= 64u ? 0ull : x >> sh;
+}
+__device__ inline uint64_t ld8z_small(const uint8_t* inb, uint32_t n, uint32_t q) {  // any n
+  uint64_t x = 0;
+  for (uint32_t t = 0; t < 8; ++t) if (q + t < n) x |= (uint64_t)inb[q + t] << (8 * t);
+  return x;
+}
+// hash5 of position q + j from the 8 bytes x at q (j < 4)
+__device__ inline uint32_t hash5_of(uint64_t x, uint32_t j) {
+  const uint32_t v = (uint32_t)(x >> (8 * j)), w = (uint32_t)(x >> (8 * j + 32)) & 0xFFu;
+  return (v * 2654435761u) ^ (w * 0x9E3779B1u + (w >> 3));
+}
+
 // ---------------------------------------------------------------- 1. hash, count, scatter
 // The key of a position is the top `keybits` bits of its hash; buckets are its top `hbits` bits. srt[b*B + slot] =
 // (the key's other keybits - hbits bits) << blog | position, grouped by bucket; bstart[b*(HB+1) + h] = the first slot
@@ -83,7 +100,7 @@ __host__ __device__ inline uint32_t pbits_of(uint32_t blog, uint32_t hbits) {
 __global__ __launch_bounds__(1024) void k_enc_hash(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
                                                    uint32_t blog, uint32_t hbits, uint32_t pbits,
                                                    uint32_t* __restrict__ tmp, uint32_t* __restrict__ bstart) {
-  extern __shared__ uint32_t cnt[];  // HB counters, HB/1024 partial sums, then the partition ranking's tables
+  extern __shared__ uint32_t cnt[];  // HB counters, 16 wavefront sums, the partition cursors, the claims
   const uint32_t b = blockIdx.x, tid = threadIdx.x;
   const uint32_t HBITS = hbits, HB = 1u << HBITS;
   const Block blk = blocks[b];
@@ -92,79 +109,80 @@ __global__ __launch_bounds__(1024) void k_enc_hash(const Block* __restrict__ blo
   const uint32_t count = n >= 4 ? n - 3 : 0;
   for (uint32_t i = tid; i < HB; i += 1024) cnt[i] = 0;
   __syncthreads();
-  for (uint32_t p = tid; p < count; p += 1024) atomicAdd(&cnt[hash5(inb, n, p) >> (32 - HBITS)], 1u);
-  __syncthreads();
-  // exclusive scan of HB counters: each thread sums a run of `per`, one thread scans the partials
-  const uint32_t per = HB / 1024u ? HB / 1024u : 1u;
-  uint32_t* part = cnt + HB;
-  if (tid * per < HB) {
-    uint32_t s = 0;
-    for (uint32_t i = 0; i < per; ++i) s += cnt[tid * per + i];
-    part[tid] = s;
+  // 4 positions per lane from one 8-byte load, the next load in flight while these are counted
+  {
+    uint64_t x = n >= 8 ? ld8z(inb, n, 4 * tid) : ld8z_small(inb, n, 4 * tid);
+    for (uint32_t q = 4 * tid; q < count; q += 4096) {
+      const uint64_t nx = n >= 8 ? ld8z(inb, n, q + 4096) : 0ull;
+#pragma unroll
+      for (uint32_t j = 0; j < 4; ++j)
+        if (q + j < count) atomicAdd(&cnt[hash5_of(x, j) >> (32 - HBITS)], 1u);
+      x = nx;
+    }
   }
   __syncthreads();
-  if (tid < 64) {  // one wavefront scans the (at most 1024) partials, 16 per lane
-    const uint32_t np = HB / per;
-    const uint32_t k = (np + 63) / 64;
+  // exclusive scan of the HB counters, HB/16 per wavefront, 64 consecutive counters per step (no LDS bank conflicts)
+  const uint32_t wv = tid >> 6, lane = tid & 63u, chunk = HB / 16u;
+  uint32_t* wsum = cnt + HB;  // [16]
+  {
     uint32_t s = 0;
-    for (uint32_t i = 0; i < k; ++i) if (tid * k + i < np) s += part[tid * k + i];
-    uint32_t run = wave_scan_excl(s);
-    for (uint32_t i = 0; i < k; ++i)
-      if (tid * k + i < np) { const uint32_t x = part[tid * k + i]; part[tid * k + i] = run; run += x; }
+    for (uint32_t i = lane; i < chunk; i += 64) s += cnt[wv * chunk + i];
+    s = wave_sum(s);
+    if (lane == 0) wsum[wv] = s;
   }
   __syncthreads();
   uint32_t* bs = bstart + (size_t)b * (HB + 1);
-  if (tid * per < HB) {
-    uint32_t run = part[tid];
-    for (uint32_t i = 0; i < per; ++i) {
-      const uint32_t x = cnt[tid * per + i];
-      cnt[tid * per + i] = run;
-      bs[tid * per + i] = run;
-      run += x;
+  {
+    uint32_t run = 0;
+    for (uint32_t x = 0; x < wv; ++x) run += wsum[x];
+    for (uint32_t i0 = 0; i0 < chunk; i0 += 64) {
+      const uint32_t i = wv * chunk + i0 + lane;
+      const bool in = i0 + lane < chunk;
+      const uint32_t c = in ? cnt[i] : 0u;
+      const uint32_t inc = wave_scan_incl(c);
+      if (in) { cnt[i] = run + inc - c; bs[i] = run + inc - c; }
+      run += __builtin_amdgcn_readlane(inc, 63);
     }
   }
   if (tid == 0) bs[HB] = count;
   __syncthreads();
-  // the positions in partition order, stable, 1024 at a time: ranks inside a wavefront from ballots over the
-  // partition's bits, then across the 16 wavefronts through LDS
+  // the positions grouped by partition (in no set order inside one: k_enc_place does not need it): for each of its 4
+  // positions, a wavefront ranks its lanes by partition from ballots over the partition's bits, and one lane per
+  // partition present claims that many slots with an LDS atomic (distinct addresses: no conflicts). No workgroup
+  // barrier inside the loop.
   const uint32_t NP = 1u << pbits;
-  uint32_t* wc = cnt + HB + 1024;  // [16][NP]
-  uint32_t* pcur = wc + 16 * NP;   // [NP] next slot of each partition
-  uint32_t* tt = pcur + NP;        // [NP] this round's count of each partition
-  for (uint32_t i = tid; i < 16 * NP; i += 1024) wc[i] = 0;
+  uint32_t* pcur = cnt + HB + 16;       // [NP] next free slot of each partition
+  uint32_t* wb = pcur + NP + wv * NP;   // [16][NP] this wavefront's claims
   if (tid < NP) pcur[tid] = cnt[tid << (HBITS - pbits)];
   __syncthreads();
   uint32_t* tb = tmp + ((size_t)b << blog);
-  const uint32_t wv = tid >> 6;
-  for (uint32_t t0 = 0; t0 < count; t0 += 1024) {
-    const uint32_t p = t0 + tid;
-    const bool valid = p < count;
-    const uint32_t d = valid && pbits ? hash5(inb, n, p) >> (32 - pbits) : 0u;
-    uint64_t m = ballot(valid);
-    for (uint32_t bit = 0; bit < pbits; ++bit) {
-      const uint64_t bb = ballot((d >> bit) & 1u);
-      m &= ((d >> bit) & 1u) ? bb : ~bb;
+  uint64_t x = n >= 8 ? ld8z(inb, n, 4 * tid) : ld8z_small(inb, n, 4 * tid);
+  for (uint32_t q = 4 * tid; q - 4 * lane < count; q += 4096) {  // (a wavefront runs while any lane has work)
+    const uint64_t nx = n >= 8 ? ld8z(inb, n, q + 4096) : 0ull;
+    uint32_t slot[4];
+#pragma unroll
+    for (uint32_t j = 0; j < 4; ++j) {
+      const bool valid = q + j < count;
+      const uint32_t d = valid && pbits ? hash5_of(x, j) >> (32 - pbits) : 0u;
+      uint64_t m = ballot(valid);
+      for (uint32_t bit = 0; bit < pbits; ++bit) {
+        const uint64_t bb = ballot((d >> bit) & 1u);
+        m &= ((d >> bit) & 1u) ? bb : ~bb;
+      }
+      const uint32_t rank = popc64(m & lanemask_lt());
+      if (valid && rank == 0) wb[d] = atomicAdd(&pcur[d], popc64(m));
+      wave_fence();
+      slot[j] = wb[d] + rank;
+      wave_fence();  // (the next claims overwrite wb)
     }
-    const uint32_t rank = popc64(m & lanemask_lt());
-    if (valid && rank == 0) wc[wv * NP + d] = popc64(m);
-    __syncthreads();
-    if (tid < NP) {
-      uint32_t run = 0;
-      for (uint32_t x = 0; x < 16; ++x) { const uint32_t c = wc[x * NP + tid]; wc[x * NP + tid] = run; run += c; }
-      tt[tid] = run;
-    }
-    __syncthreads();
-    if (valid) tb[pcur[d] + wc[wv * NP + d] + rank] = p;
-    __syncthreads();
-    if (tid < NP) {
-      pcur[tid] += tt[tid];
-      for (uint32_t x = 0; x < 16; ++x) wc[x * NP + tid] = 0;
-    }
-    __syncthreads();
+#pragma unroll
+    for (uint32_t j = 0; j < 4; ++j)
+      if (q + j < count) tb[slot[j]] = q + j;
+    x = nx;
   }
 }
 
-// one workgroup per (block, partition): the partition's positions (in position order, from tmp) to their buckets
+// one workgroup per (block, partition): the partition's positions (from tmp) to their buckets
 __global__ __launch_bounds__(1024) void k_enc_place(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
                                                     uint32_t nb, uint32_t blog, uint32_t hbits, uint32_t keybits,
                                                     uint32_t pbits, const uint32_t* __restrict__ tmp,
@@ -187,13 +205,22 @@ __global__ __launch_bounds__(1024) void k_enc_place(const Block* __restrict__ bl
   const uint32_t* tb = tmp + ((size_t)b << blog) + ps;
   uint32_t* sb = srt + ((size_t)b << blog) + ps;
   const uint32_t tagmask = (1u << (keybits - hbits)) - 1u;
-  for (uint32_t i = tid; i < m; i += 1024) {
-    const uint32_t p = tb[i];
-    const uint32_t h = hash5(inb, n, p);
-    const uint32_t slot = atomicAdd(&lc[(h >> (32 - hbits)) & (NB - 1u)], 1u);
-    const uint32_t e = ((h >> (32 - keybits)) & tagmask) << blog | p;
-    if (in_lds) buf[slot] = e;
-    else sb[slot] = e;
+  for (uint32_t i0 = tid; i0 < m; i0 += 4096) {  // 4 entries per lane, every load issued before any is used
+    uint32_t p[4];
+    uint64_t x[4];
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) p[u] = tb[min(i0 + 1024 * u, m - 1)];
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) x[u] = n >= 8 ? ld8z(inb, n, p[u]) : ld8z_small(inb, n, p[u]);
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u)
+      if (i0 + 1024 * u < m) {
+        const uint32_t h = hash5_of(x[u], 0);
+        const uint32_t slot = atomicAdd(&lc[(h >> (32 - hbits)) & (NB - 1u)], 1u);
+        const uint32_t e = ((h >> (32 - keybits)) & tagmask) << blog | p[u];
+        if (in_lds) buf[slot] = e;
+        else sb[slot] = e;
+      }
   }
   __syncthreads();
   if (in_lds)

@@ -48,7 +48,7 @@ size_t parse_lds_bytes(const EncShape& S) {
 }
 bool shape_fits(const EncShape& S) {
   const size_t np = (size_t)1 << pbits_of(S.blog, S.hbits);
-  return shape_ok(S) && parse_lds_bytes(S) <= 160 * 1024 && (((size_t)1 << S.hbits) + 1024 + 18 * np) * 4 <= 160 * 1024 &&
+  return shape_ok(S) && parse_lds_bytes(S) <= 160 * 1024 && (((size_t)1 << S.hbits) + 16 + 17 * np) * 4 <= 160 * 1024 &&
          (((size_t)1 << (S.hbits - pbits_of(S.blog, S.hbits))) + PLACE_CAP) * 4 <= 160 * 1024;
 }
 
