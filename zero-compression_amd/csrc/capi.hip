@@ -119,10 +119,11 @@ struct EncLayout {
 struct EncShape {
   uint32_t blog, hbits, keybits, walkers;
 };
-EncShape default_shape(uint32_t blog) { return {blog, hbits_of(blog), hbits_of(blog) + KEY_TAG_BITS, 256}; }
+uint32_t walkers_of(uint32_t blog) { return std::min<uint32_t>(PARSE_WALKERS, (1u << blog) / 32); }
+EncShape default_shape(uint32_t blog) { return {blog, hbits_of(blog), hbits_of(blog) + KEY_TAG_BITS, walkers_of(blog)}; }
 bool shape_ok(const EncShape& S) {
   return S.blog >= 10 && S.blog <= 20 && S.hbits >= 4 && S.hbits <= 15 && S.hbits <= S.keybits &&
-         S.keybits <= 27 && S.keybits - S.hbits + S.blog <= 32 && S.walkers >= 1 && S.walkers <= 256 &&
+         S.keybits <= 27 && S.keybits - S.hbits + S.blog <= 32 && S.walkers >= 1 && S.walkers <= 1024 &&
          (S.walkers & (S.walkers - 1)) == 0 && ((1u << S.blog) / S.walkers) >= 32;
 }
 
@@ -376,8 +377,8 @@ static int encode_run(zgpu_t* g, const uint8_t* d_in, const Block* blocks, uint3
   if (prof) stage(g, st, "match");
   k_enc_lazy<<<posgroups, 256, 0, st>>>(blocks, blog, mt, srt);
   if (prof) stage(g, st, "lazy");
-  const size_t parse_lds = ((size_t)G.B / 32 + 256 + G.B / S.walkers) * 4;
-  k_enc_parse<<<nb, 256, parse_lds, st>>>(blocks, d_in, blog, S.walkers, lencap, srt, (uint32_t*)(base + L.mark));
+  const size_t parse_lds = ((size_t)G.B / 32 + 19 * S.walkers) * 4;
+  k_enc_parse<<<nb, std::max(64u, S.walkers), parse_lds, st>>>(blocks, d_in, blog, S.walkers, lencap, srt, (uint32_t*)(base + L.mark));
   if (prof) stage(g, st, "parse");
   k_enc_tokens<<<nb, 64, 0, st>>>(blocks, d_in, blog, srt, (uint32_t*)(base + L.mark), T);
   if (prof) stage(g, st, "tokens");

@@ -20,6 +20,7 @@ namespace zk {
 
 #define KEY_TAG_BITS 12u
 __host__ __device__ inline uint32_t hbits_of(uint32_t blog) { return blog - 3u < 15u ? blog - 3u : 15u; }
+constexpr uint32_t PARSE_WALKERS = 256;  // parse segments per block (at most; each at least 32 positions)
 
 __device__ inline uint32_t ld32(const uint8_t* p) { uint32_t v; __builtin_memcpy(&v, p, 4); return v; }
 __device__ inline uint64_t ld64(const uint8_t* p) { uint64_t v; __builtin_memcpy(&v, p, 8); return v; }
@@ -548,79 +549,157 @@ __device__ inline uint32_t match_at(uint32_t v, uint32_t p, uint32_t n) {
   return l >= 4 && l <= n - p && d <= p ? l : 0u;
 }
 
+// prefix_len for long matches: 64 bytes a step, all 8 loads of a step in flight together
+__device__ inline uint32_t prefix_len_long(const uint8_t* inb, uint32_t a, uint32_t b, uint32_t maxlen) {
+  uint32_t l = 0;
+  while (l + 64 <= maxlen) {
+    uint64_t x[8], y[8];
+#pragma unroll
+    for (uint32_t u = 0; u < 8; u += 2) {
+      __builtin_memcpy(&x[u], inb + a + l + 8 * u, 16);
+      __builtin_memcpy(&y[u], inb + b + l + 8 * u, 16);
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < 8; ++u)
+      if (x[u] != y[u]) return l + 8 * u + ((uint32_t)__builtin_ctzll(x[u] ^ y[u]) >> 3);
+    l += 64;
+  }
+  return l + prefix_len(inb, a + l, b + l, maxlen - l);
+}
+
 // the position after the token at p; a match cut at LENCAP is finished here, up to 4095 bytes (kernels/zero.cl:259-271)
-__device__ inline uint32_t nextpos(uint32_t* m, const uint8_t* inb, uint32_t n, uint32_t lencap, uint32_t p) {
-  const uint32_t v = m[p];
+__device__ inline uint32_t nextpos_v(uint32_t* m, const uint8_t* inb, uint32_t n, uint32_t lencap, uint32_t p,
+                                     uint32_t v) {  // v = m[p]
   uint32_t l = match_at(v, p, n);
   if (l == lencap) {
     const uint32_t maxlen = min(4095u, n - p);
     if (maxlen > l) {
       const uint32_t j = p - (v & 0xFFFFFu) - 1u;
-      const uint32_t x = l + prefix_len(inb, j + l, p + l, maxlen - l);
+      const uint32_t x = l + prefix_len_long(inb, j + l, p + l, maxlen - l);
       if (x != l) { m[p] = (x << 20) | (v & 0xFFFFFu); l = x; }
     }
   }
   return p + (l ? l : 1u);
 }
+__device__ inline uint32_t nextpos(uint32_t* m, const uint8_t* inb, uint32_t n, uint32_t lencap, uint32_t p) {
+  return nextpos_v(m, inb, n, lencap, p, m[p]);
+}
 
-// Each of the first `walkers` threads (a power of two, 32 <= B / walkers) walks its segment of B / walkers positions
-// from the segment's start, marking token starts in an LDS bitmap.
-// Thread 0 then follows the true chain: where it meets a walk the two coincide from there on (the chain is a pure
-// function of the position), so only the walk's prefix is wrong; where it never meets one, the walk is wrong whole.
+// sets the bits a .. b - 1 of the bitmap bm (a < b <= a + 32)
+__device__ inline void mark_range(uint32_t* bm, uint32_t a, uint32_t b) {
+  const uint64_t bits = (b - a == 64u ? ~0ull : ((1ull << (b - a)) - 1ull)) << (a & 31u);
+  bm[a >> 5] |= (uint32_t)bits;
+  if (((b - 1u) >> 5) != (a >> 5)) bm[(a >> 5) + 1u] |= (uint32_t)(bits >> 32);
+}
+
+// the chain from e through the positions before send: where it meets a mark in bm, that position | 1 << 31; else
+// the first position at or past send
+__device__ inline uint32_t join_walk(uint32_t* m, const uint8_t* inb, uint32_t n, uint32_t lencap, const uint32_t* bm,
+                                     uint32_t e, uint32_t send) {
+  uint32_t t = e;
+  while (t < send) {
+    if ((bm[t >> 5] >> (t & 31u)) & 1u) return t | 0x80000000u;
+    t = nextpos(m, inb, n, lencap, t);
+  }
+  return t;
+}
+
+// Each of the first `walkers` threads (a power of two, 32 <= B / walkers <= 1024) walks its segment of B / walkers positions
+// from the segment's start, marking token starts in an LDS bitmap; the join (below) puts the true chain in their
+// place: where the chain meets a walk the two coincide from there on (the chain is a pure function of the position),
+// so only the walk's prefix is wrong; where it never meets one, the walk is wrong whole.
 // mark[b] receives the bitmap of the chain (B bits).
-__global__ __launch_bounds__(256) void k_enc_parse(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
-                                                   uint32_t blog, uint32_t walkers, uint32_t lencap,
-                                                   uint32_t* __restrict__ lazy, uint32_t* __restrict__ mark) {
-  extern __shared__ uint32_t sm[];  // bitmap (B/32 words), then ex[256], then the join's path (SEG entries)
-  const uint32_t B = 1u << blog, SEG = B / walkers;
+__global__ __launch_bounds__(1024) void k_enc_parse(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
+                                                    uint32_t blog, uint32_t walkers, uint32_t lencap,
+                                                    uint32_t* __restrict__ lazy, uint32_t* __restrict__ mark) {
+  extern __shared__ uint32_t sm[];  // bitmap (B/32 words), ex, sin, sout (walkers words each), windows (16 each)
+  const uint32_t B = 1u << blog, SEG = B / walkers, T = blockDim.x;  // (T >= walkers)
   const uint32_t b = blockIdx.x, k = threadIdx.x;
   uint32_t* bm = sm;
   uint32_t* ex = sm + B / 32u;
-  uint32_t* path = ex + 256;
+  uint32_t* sin = ex + walkers;    // a segment's entry as walked
+  uint32_t* sout = sin + walkers;  // where that walk met walk q (| 1 << 31), or left the segment
   const Block blk = blocks[b];
   const uint8_t* inb = in + blk.src;
   const uint32_t n = blk.raw_len;
   uint32_t* m = lazy + ((size_t)b << blog);
   if (blk.level) { uint32_t nc; level_params(blk.level, nc, lencap); }
-  for (uint32_t i = k; i < B / 32u; i += 256) bm[i] = 0;
+  for (uint32_t i = k; i < B / 32u; i += T) bm[i] = 0;
   __syncthreads();
   if (k < walkers) {
+    // m is read 16 entries (64 bytes) at a time into this walker's LDS window: the many walks in flight would
+    // otherwise fetch a line of m for each step, their lines leaving the caches before the next step comes
+    // A step goes from p straight to the window's next position with a match (the positions before it are
+    // literals, one chain position each) or, with none, to the window's end.
+    uint32_t* win = sout + walkers + 16 * k;
     const uint32_t s = k * SEG, send = min(s + SEG, n);
-    uint32_t p = s;
+    uint32_t p = s, wb = s - 16u, mm = 0;  // window base (none yet), which of its positions have a match
     while (p < send) {
-      bm[p >> 5] |= 1u << (p & 31u);  // (a segment owns whole words: SEG is a multiple of 32)
-      p = nextpos(m, inb, n, lencap, p);
+      if (p - wb >= 16u) {
+        wb = p & ~15u;
+        uint32_t v[16];
+        __builtin_memcpy(v, m + wb, 64);
+        mm = 0;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          win[q] = v[q];
+          mm |= (match_at(v[q], wb + q, n) ? 1u : 0u) << q;
+        }
+      }
+      const uint32_t ahead = mm & (~0u << (p - wb));
+      const uint32_t q = ahead ? wb + (uint32_t)__builtin_ctz(ahead) : wb + 16u;
+      if (q >= send) {  // literals to the segment's end
+        mark_range(bm, p, send);
+        p = send;
+      } else if (!ahead) {
+        mark_range(bm, p, q);
+        p = q;
+      } else {
+        mark_range(bm, p, q + 1);  // (a segment owns whole words: SEG is a multiple of 32)
+        p = nextpos_v(m, inb, n, lencap, q, win[q - wb]);
+      }
     }
     ex[k] = s < n ? p : s;
   }
   __syncthreads();
+  // the join. Segment q's true entry is where the chain of the segments before it leaves them; from there the chain
+  // runs until it meets a mark of walk q (from then on the two coincide, and the chain leaves the segment at ex[q])
+  // or leaves the segment. Each segment first assumes its entry to be ex[q - 1] (right whenever the chain met walk
+  // q - 1) and walks from it, all segments at once; thread 0 then follows the entries in order and walks again only
+  // where an assumption was wrong; last, each segment clears the walk's marks before the meeting point and marks the
+  // chain's own positions there.
+  const uint32_t nseg = min(walkers, (n + SEG - 1) / SEG);
+  if (k < nseg) {
+    const uint32_t e = k ? ex[k - 1] : 0u;
+    sin[k] = e;
+    sout[k] = join_walk(m, inb, n, lencap, bm, e, min(k * SEG + SEG, n));
+  }
+  __syncthreads();
   if (k == 0) {
     uint32_t e = 0;
-    for (uint32_t q = 0; q < walkers; ++q) {
-      const uint32_t s = q * SEG;
-      if (s >= n) break;
-      const uint32_t send = min(s + SEG, n);
-      uint32_t clear_to = send, np = 0, t = e;
-      if (e < send) {
-        bool merged = false;
-        while (t < send) {
-          if ((bm[t >> 5] >> (t & 31u)) & 1u) { merged = true; break; }
-          path[np++] = t;
-          t = nextpos(m, inb, n, lencap, t);
-        }
-        if (merged) { clear_to = t; e = ex[q]; } else { e = t; }
+    for (uint32_t q = 0; q < nseg; ++q) {
+      uint32_t r = sout[q];
+      if (sin[q] != e) {
+        r = join_walk(m, inb, n, lencap, bm, e, min(q * SEG + SEG, n));
+        sin[q] = e;
+        sout[q] = r;
       }
-      // the walk's marks before the meeting point are not the chain's
-      for (uint32_t x = s; x < clear_to;) {
-        if ((x & 31u) == 0 && x + 32 <= clear_to) { bm[x >> 5] = 0; x += 32; }
-        else { bm[x >> 5] &= ~(1u << (x & 31u)); ++x; }
-      }
-      for (uint32_t i = 0; i < np; ++i) bm[path[i] >> 5] |= 1u << (path[i] & 31u);
+      e = r >> 31 ? ex[q] : r;
     }
   }
   __syncthreads();
+  if (k < nseg) {
+    const uint32_t s = k * SEG, send = min(s + SEG, n), r = sout[k];
+    const uint32_t clear_to = r >> 31 ? r & 0x7FFFFFFFu : send;
+    for (uint32_t x = s; x < clear_to;) {
+      if ((x & 31u) == 0 && x + 32 <= clear_to) { bm[x >> 5] = 0; x += 32; }
+      else { bm[x >> 5] &= ~(1u << (x & 31u)); ++x; }
+    }
+    for (uint32_t p = sin[k]; p < clear_to; p = nextpos(m, inb, n, lencap, p)) bm[p >> 5] |= 1u << (p & 31u);
+  }
+  __syncthreads();
   uint32_t* mk = mark + ((size_t)b << (blog - 5));
-  for (uint32_t i = k; i < B / 32u; i += 256) mk[i] = bm[i];
+  for (uint32_t i = k; i < B / 32u; i += T) mk[i] = bm[i];
 }
 
 // ---------------------------------------------------------------- 6. tokens and streams (one wavefront per block)

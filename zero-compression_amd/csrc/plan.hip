@@ -39,12 +39,12 @@ EncShape class_shape(uint32_t blog, uint32_t key_log) {
   S.keybits = hbits_of(key_log) + KEY_TAG_BITS;
   S.hbits = std::min(S.hbits, S.keybits);
   while (S.keybits - S.hbits + S.blog > 32) ++S.hbits;
-  S.walkers = std::min<uint32_t>(256, (1u << blog) / 32);
+  S.walkers = walkers_of(blog);
   return S;
 }
 size_t parse_lds_bytes(const EncShape& S) {
   const size_t B = (size_t)1 << S.blog;
-  return (B / 32 + 256 + B / S.walkers) * 4;
+  return (B / 32 + 19 * (size_t)S.walkers) * 4;
 }
 bool shape_fits(const EncShape& S) {
   const size_t np = (size_t)1 << pbits_of(S.blog, S.hbits);
@@ -310,7 +310,7 @@ int zgpu_plan_tune(zgpu_plan_t* p, const uint8_t* d_in, int level, uint8_t* d_sl
     EncShape best = c.shape;
     float best_ms = 1e30f;
     for (int dh = -2; dh <= 1 && rc == ZGPU_OK; ++dh) {
-      for (uint32_t w : {64u, 128u, 256u}) {
+      for (uint32_t w : {128u, 256u, 512u, 1024u}) {
         EncShape S = base;
         S.hbits = (uint32_t)((int)base.hbits + dh);
         S.walkers = w;
