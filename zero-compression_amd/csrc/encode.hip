@@ -736,23 +736,47 @@ __global__ __launch_bounds__(64) void k_enc_tokens(const Block* __restrict__ blo
   uint32_t* LB = T.LB + (size_t)b * G.TCAP;
   uint32_t* TM = T.TM + (size_t)b * G.TCAP;
   uint32_t* TD = T.TD + (size_t)b * G.TCAP;
-  // phase A: the chain in 64-position chunks
+  // phase A: the chain, 512 positions a step, 8 consecutive ones per lane (ranks from one scan per step); every load
+  // of the next step is in flight while this one is written
   uint32_t nt = 0, nl = 0;
-  for (uint32_t p0 = 0; p0 < n; p0 += 64) {
-    const uint64_t w = (uint64_t)mk[p0 >> 5] | (uint64_t)mk[(p0 >> 5) + 1] << 32;
-    if (!w) continue;
-    const uint32_t p = p0 + lane;
-    const bool on = p < n && ((w >> lane) & 1u);
-    const uint32_t v = on ? m[p] : 0u;
-    const bool istok = on && match_at(v, p, n) != 0;
-    const bool islit = on && !istok;
-    const uint64_t lm = ballot(islit), tm = ballot(istok);
-    const uint32_t lq = nl + popc64(lm & lanemask_lt());
-    const uint32_t tq = nt + popc64(tm & lanemask_lt());
-    if (islit) lit[lq] = inb[p];
-    if (istok) { LB[tq] = lq; TM[tq] = v >> 20; TD[tq] = (v & 0xFFFFFu) + 1u; }
-    nl += popc64(lm);
-    nt += popc64(tm);
+  const bool big = n >= 8;
+  uint32_t cm[8], nm[8];
+  uint64_t cx, nx = 0;
+  uint32_t cw, nw = 0;
+  __builtin_memcpy(cm, m + 8 * lane, 32);
+  cx = big ? ld8z(inb, n, 8 * lane) : ld8z_small(inb, n, 8 * lane);
+  cw = mk[lane >> 2];
+  for (uint32_t p0 = 0; p0 < n; p0 += 512) {
+    const uint32_t qn = p0 + 512 + 8 * lane;
+    if (p0 + 512 < n) {  // (uniform)
+      __builtin_memcpy(nm, m + qn, 32);
+      nx = ld8z(inb, n, qn);
+      nw = mk[qn >> 5];
+    }
+    const uint32_t base = p0 + 8 * lane;
+    const uint32_t bits = (cw >> (base & 31u)) & 0xFFu;  // which of this lane's 8 positions are on the chain
+    uint32_t tmask = 0;
+#pragma unroll
+    for (uint32_t u = 0; u < 8; ++u)
+      if (((bits >> u) & 1u) && match_at(cm[u], base + u, n)) tmask |= 1u << u;
+    const uint32_t lmask = bits & ~tmask;
+    const uint32_t lc = (uint32_t)__builtin_popcount(lmask), tc = (uint32_t)__builtin_popcount(tmask);
+    const uint32_t li = wave_scan_incl(lc), ti = wave_scan_incl(tc);
+    uint32_t lq = nl + li - lc, tq = nt + ti - tc;
+#pragma unroll
+    for (uint32_t u = 0; u < 8; ++u) {
+      if ((lmask >> u) & 1u) lit[lq++] = (uint8_t)(cx >> (8u * u));
+      if ((tmask >> u) & 1u) {
+        LB[tq] = lq; TM[tq] = cm[u] >> 20; TD[tq] = (cm[u] & 0xFFFFFu) + 1u;
+        ++tq;
+      }
+    }
+    nl += readlane(li, 63);
+    nt += readlane(ti, 63);
+#pragma unroll
+    for (uint32_t u = 0; u < 8; ++u) cm[u] = nm[u];
+    cx = nx;
+    cw = nw;
   }
   wave_fence();
   // phase B: 64 tokens at a time
@@ -763,15 +787,23 @@ __global__ __launch_bounds__(64) void k_enc_tokens(const Block* __restrict__ blo
   uint32_t* ow = T.offw + (size_t)b * (G.OCAP / 4u + 1u);
   uint32_t op = 0, ei = 0, bp = 0, ng = 0, gstart = 0, gt = 0, carry = 0;
   uint32_t r0 = 1, r1 = 4, r2 = 8, l0 = 1, l1 = 4, l2 = 8;
+  // (each chunk's three loads are issued one chunk ahead, at clamped indexes)
+  uint32_t lbr = 0, Mr = 0, Dr = 1, lbprev = 0;
+  if (nt) { const uint32_t q = min(lane, nt - 1); lbr = LB[q]; Mr = TM[q]; Dr = TD[q]; }
   for (uint32_t t = 0; t < nt; t += 64) {
     const uint32_t c = min(64u, nt - t);
     const bool on = lane < c;
     const uint32_t q = t + lane;
-    const uint32_t lb = on ? LB[q] : 0u;
-    const uint32_t lprev = on && q ? LB[q - 1] : 0u;
+    const uint32_t qn = min(t + 64 + lane, nt - 1);
+    const uint32_t lbn = LB[qn], Mn = TM[qn], Dn = TD[qn];
+    const uint32_t lb = on ? lbr : 0u;
+    const uint32_t up = (uint32_t)__shfl_up((int)lbr, 1, 64);
+    const uint32_t lprev = on ? (lane ? up : lbprev) : 0u;
     const uint32_t L = lb - lprev;
-    const uint32_t M = on ? TM[q] : 0u;
-    const uint32_t D = on ? TD[q] : 1u;
+    const uint32_t M = on ? Mr : 0u;
+    const uint32_t D = on ? Dr : 1u;
+    lbprev = readlane(lbr, 63);
+    lbr = lbn; Mr = Mn; Dr = Dn;
     const uint32_t opk = op + wave_scan_excl(L + M);
     const uint32_t i = opk + L, j = i - D;
     const uint32_t need = j < opk ? j + min(M, D) : 0u;  // how far the source reaches, unless its own literals
