@@ -866,6 +866,14 @@ __global__ __launch_bounds__(64) void k_enc_tokens(const Block* __restrict__ blo
 }
 
 // ---------------------------------------------------------------- 7. rANS per (block, stream)
+constexpr uint32_t RING_STRIDE = 144;  // a lane's LDS ring of 128 bytes, 8 of spill, padded to 16
+// 64 bytes from LDS (16-byte aligned) to memory (64-byte aligned)
+__device__ inline void copy64(uint8_t* dst, const uint8_t* src) {
+  const uint4* s = reinterpret_cast<const uint4*>(src);
+  uint4* d = reinterpret_cast<uint4*>(dst);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) d[q] = s[q];
+}
 struct RansBufs {
   uint8_t* lanes;     // lane scratch, LANES bytes per block
   uint16_t* freq;     // 256 per (block, stream)
@@ -874,7 +882,8 @@ struct RansBufs {
 };
 __host__ __device__ inline uint32_t lane_cap(const Geo& G, uint32_t s) {
   const uint32_t maxn = s == 0 ? G.B : s == 2 ? G.ECAP : G.TCAP;
-  return 2u * ((maxn + 63u) / 64u) + 16u;  // at most 2 bytes per symbol, plus the 4-byte state
+  // at most 2 bytes per symbol, plus the 4-byte state; a multiple of 64 (lanes leave in aligned 64-byte pieces)
+  return (2u * ((maxn + 63u) / 64u) + 16u + 63u) & ~63u;
 }
 __host__ __device__ inline uint32_t lane_off(const Geo& G, uint32_t s) {
   uint32_t o = 0;
@@ -977,13 +986,20 @@ __global__ __launch_bounds__(64) void k_enc_rans(uint32_t blog, TokBufs T, RansB
   __syncthreads();
   // lane j encodes symbols j, j+64, ... from the last one back into its scratch, backwards (src/codec.h:112-126).
   // Steps go 4 at a time: the 256 symbols of 4 steps arrive in LDS from one coalesced load made a group earlier,
-  // and the (at most 8) bytes the lane emits in them leave in one 8-byte store ending where its bytes stand (the
-  // bytes below them in that store are garbage that the next group's store covers). Every group makes exactly one
-  // load and one store, so a wait for the load never has to wait for the stores after it.
+  // and the (at most 8) bytes the lane emits in them go, as one 8-byte write ending where its bytes stand (the bytes
+  // below them are garbage that the next group's write covers), into the lane's ring of 128 bytes in LDS. Every 4
+  // groups (at most 32 bytes) a lane whose next aligned 64-byte piece is complete stores it whole: no partial line
+  // of the scratch goes to memory, and most groups make one load and no store. (Between two such points the lane
+  // holds at most 63 + 32 bytes not yet stored, and writes 8 below them: the ring never overtakes them.)
   __shared__ uint32_t stage[64];
+  __shared__ __align__(16) uint8_t rings[64 * RING_STRIDE];
+  uint8_t* rl = rings + lane * RING_STRIDE;  // byte at address a: rl[(a - end) mod 128], with 8 bytes of spill past 128
   const uint32_t LC = lane_cap(G, s);
   uint8_t* end = R.lanes + (size_t)b * lanes_per_block(G) + lane_off(G, s) + (size_t)(lane + 1) * LC;
   uint8_t* ptr = end;
+  uint8_t* fl = end;     // [fl, end) is in memory; fl is 64-byte aligned
+  uint32_t ro = 120;     // ring offset of ptr - 8
+  uint32_t fro = 64;     // ring offset of fl - 64
   uint32_t x = 1u << 23;
   const uint32_t NT = (n + 63u) / 64u;                 // steps: symbol lane + 64 t at step t, t from NT-1 down
   int32_t j = (int32_t)((NT - 1u) >> 2);               // groups of steps [4j, 4j + 3]
@@ -1020,11 +1036,29 @@ __global__ __launch_bounds__(64) void k_enc_rans(uint32_t blog, TokBufs T, RansB
       const uint32_t xn = (qt << 12) + (x2 - qt * fq) + (e >> 16);
       x = valid ? xn : x;
     }
-    __builtin_memcpy(ptr - 8, &acc, 8);
+    __builtin_memcpy(rl + ro, &acc, 8);
+    if (ro > 120u) {  // (the part past the ring's end belongs at its start)
+      const uint32_t over = ro - 120u;  // bytes past 128: 1..7
+      const uint64_t hi = acc >> (8u * (8u - over));
+      for (uint32_t i = 0; i < over; ++i) rl[i] = (uint8_t)(hi >> (8u * i));
+    }
     ptr -= k;
+    ro = (ro - k) & 127u;
+    if ((j & 3) == 0 && ptr + 64 <= fl) {  // the piece [fl - 64, fl) is final
+      copy64(fl - 64, rl + fro);
+      fl -= 64;
+      fro = (fro - 64u) & 127u;
+    }
   }
   ptr -= 4;
-  ptr[0] = (uint8_t)x; ptr[1] = (uint8_t)(x >> 8); ptr[2] = (uint8_t)(x >> 16); ptr[3] = (uint8_t)(x >> 24);
+  ro = (ro - 4u) & 127u;  // (the ring offset of ptr - 8 again)
+#pragma unroll
+  for (uint32_t i = 0; i < 4; ++i) rl[(ro + 8u + i) & 127u] = (uint8_t)(x >> (8u * i));  // the state at ptr[0..3]
+  while (ptr < fl) {  // the rest, in whole pieces (below ptr: unused scratch of this lane)
+    copy64(fl - 64, rl + fro);
+    fl -= 64;
+    fro = (fro - 64u) & 127u;
+  }
   const uint32_t ll = (uint32_t)(end - ptr);
   R.lanelen[(size_t)g * 64 + lane] = ll;
   const uint32_t total = 32u + 2u * npres + 256u + wave_sum(ll);
