@@ -375,12 +375,10 @@ static int encode_run(zgpu_t* g, const uint8_t* d_in, const Block* blocks, uint3
   if (prof) stage(g, st, "sort");
   k_enc_match<<<posgroups, 256, 0, st>>>(blocks, d_in, blog, S.keybits, ncand, lencap, srt, mt);
   if (prof) stage(g, st, "match");
-  k_enc_lazy<<<posgroups, 256, 0, st>>>(blocks, blog, mt, srt);
-  if (prof) stage(g, st, "lazy");
   const size_t parse_lds = ((size_t)G.B / 32 + 19 * S.walkers) * 4;
-  k_enc_parse<<<nb, std::max(64u, S.walkers), parse_lds, st>>>(blocks, d_in, blog, S.walkers, lencap, srt, (uint32_t*)(base + L.mark));
+  k_enc_parse<<<nb, std::max(64u, S.walkers), parse_lds, st>>>(blocks, d_in, blog, S.walkers, lencap, mt, (uint32_t*)(base + L.mark));
   if (prof) stage(g, st, "parse");
-  k_enc_tokens<<<nb, 64, 0, st>>>(blocks, d_in, blog, srt, (uint32_t*)(base + L.mark), T);
+  k_enc_tokens<<<nb, 64, 0, st>>>(blocks, d_in, blog, lencap, mt, (uint32_t*)(base + L.mark), T);
   if (prof) stage(g, st, "tokens");
   k_enc_rans<<<nb * 5, 64, 0, st>>>(blog, T, R);
   if (prof) stage(g, st, "rans");

@@ -9,9 +9,8 @@
 //   k_enc_sort_m    queued buckets of 65..4096 entries: bitonic sort in LDS
 //   k_enc_sort_l    larger buckets: a workgroup's stable LSD radix sort through global memory
 //   k_enc_match     the best match of every position among its run check and sorted neighbours
-//   k_enc_lazy      a match gives way to a better one a byte later
-//   k_enc_parse     the greedy chain: segment walkers in parallel, then one lane joins them
-//   k_enc_tokens    the chain -> literals and tokens; groups, recent offsets, tok/ext/sym/groups, offset bits
+//   k_enc_parse     the greedy chain (with lazy matching): segment walkers in parallel, then a join of their walks
+//   k_enc_tokens    the chain -> literals and tokens (lazy values again); groups, recent offsets, tok/ext/sym/groups, offset bits
 //   k_enc_rans      per (block, stream): histogram, 12-bit frequencies, 64 lanes of rANS
 //   k_enc_pack      the payload, or "stored" when it is not smaller than the block
 #include "zcommon.h"
@@ -521,24 +520,17 @@ __global__ __launch_bounds__(256) void k_enc_match(const Block* __restrict__ blo
 }
 
 // ---------------------------------------------------------------- 4. lazy matching (kernels/zero.cl:239-255)
-__global__ __launch_bounds__(256) void k_enc_lazy(const Block* __restrict__ blocks, uint32_t blog,
-                                                  const uint32_t* __restrict__ match, uint32_t* __restrict__ lazy) {
-  const uint32_t B = 1u << blog;
-  const uint32_t wg = xcd_swizzle(blockIdx.x, gridDim.x);
-  const uint32_t b = wg >> (blog - 8);
-  const uint32_t p = ((wg << 8) + threadIdx.x) & (B - 1u);
-  const uint32_t n = blocks[b].raw_len;
-  if (p >= n) return;
-  const size_t at = ((size_t)b << blog) + p;
-  const uint32_t v = match[at];
-  const uint32_t v2 = p + 1 < n ? match[at + 1] : 0u;
-  const int l = (int)(v >> 20), l2 = (int)(v2 >> 20);
-  uint32_t r = v;
+// A match gives way to a better one a byte later: the value the parse sees at p, from the match finder's values at
+// p and p + 1 (0 past the block). It is computed where it is used, by the parse and the token pass. The parse
+// writes a match it finishes past LENCAP back into the match array; lengths here are taken at most LENCAP, and a
+// finished match was LENCAP long before, so the decisions come out the same before and after.
+__device__ inline uint32_t lazy_of(uint32_t v, uint32_t v2, uint32_t lencap) {
+  const uint32_t l = min(v >> 20, lencap), l2 = min(v2 >> 20, lencap);
   if (l && l2) {
     const int c = (int)bitlen((v & 0xFFFFFu) + 1u), c2 = (int)bitlen((v2 & 0xFFFFFu) + 1u);
-    if (8 * l2 - c2 - 8 > 8 * l - c) r = 0;
+    if (8 * (int)l2 - c2 - 8 > 8 * (int)l - c) return 0u;
   }
-  lazy[at] = r;
+  return v;
 }
 
 // ---------------------------------------------------------------- 5. the greedy parse (one workgroup per block)
@@ -582,7 +574,7 @@ __device__ inline uint32_t nextpos_v(uint32_t* m, const uint8_t* inb, uint32_t n
   return p + (l ? l : 1u);
 }
 __device__ inline uint32_t nextpos(uint32_t* m, const uint8_t* inb, uint32_t n, uint32_t lencap, uint32_t p) {
-  return nextpos_v(m, inb, n, lencap, p, m[p]);
+  return nextpos_v(m, inb, n, lencap, p, lazy_of(m[p], p + 1 < n ? m[p + 1] : 0u, lencap));
 }
 
 // sets the bits a .. b - 1 of the bitmap bm (a < b <= a + 32)
@@ -611,7 +603,7 @@ __device__ inline uint32_t join_walk(uint32_t* m, const uint8_t* inb, uint32_t n
 // mark[b] receives the bitmap of the chain (B bits).
 __global__ __launch_bounds__(1024) void k_enc_parse(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
                                                     uint32_t blog, uint32_t walkers, uint32_t lencap,
-                                                    uint32_t* __restrict__ lazy, uint32_t* __restrict__ mark) {
+                                                    uint32_t* __restrict__ match, uint32_t* __restrict__ mark) {
   extern __shared__ uint32_t sm[];  // bitmap (B/32 words), ex, sin, sout (walkers words each), windows (16 each)
   const uint32_t B = 1u << blog, SEG = B / walkers, T = blockDim.x;  // (T >= walkers)
   const uint32_t b = blockIdx.x, k = threadIdx.x;
@@ -622,7 +614,7 @@ __global__ __launch_bounds__(1024) void k_enc_parse(const Block* __restrict__ bl
   const Block blk = blocks[b];
   const uint8_t* inb = in + blk.src;
   const uint32_t n = blk.raw_len;
-  uint32_t* m = lazy + ((size_t)b << blog);
+  uint32_t* m = match + ((size_t)b << blog);
   if (blk.level) { uint32_t nc; level_params(blk.level, nc, lencap); }
   for (uint32_t i = k; i < B / 32u; i += T) bm[i] = 0;
   __syncthreads();
@@ -637,13 +629,15 @@ __global__ __launch_bounds__(1024) void k_enc_parse(const Block* __restrict__ bl
     while (p < send) {
       if (p - wb >= 16u) {
         wb = p & ~15u;
-        uint32_t v[16];
+        uint32_t v[17];
         __builtin_memcpy(v, m + wb, 64);
+        v[16] = m[min(wb + 16u, B - 1u)];  // (used only when wb + 16 < n)
         mm = 0;
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
-          win[q] = v[q];
-          mm |= (match_at(v[q], wb + q, n) ? 1u : 0u) << q;
+          const uint32_t lz = lazy_of(v[q], wb + q + 1u < n ? v[q + 1] : 0u, lencap);
+          win[q] = lz;
+          mm |= (match_at(lz, wb + q, n) ? 1u : 0u) << q;
         }
       }
       const uint32_t ahead = mm & (~0u << (p - wb));
@@ -722,7 +716,8 @@ struct TokBufs {
 };
 
 __global__ __launch_bounds__(64) void k_enc_tokens(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
-                                                   uint32_t blog, const uint32_t* __restrict__ lazy,
+                                                   uint32_t blog, uint32_t lencap,
+                                                   const uint32_t* __restrict__ match,
                                                    const uint32_t* __restrict__ mark, TokBufs T) {
   __shared__ uint32_t stage[48];
   const Geo G = geo_of(blog);
@@ -730,7 +725,8 @@ __global__ __launch_bounds__(64) void k_enc_tokens(const Block* __restrict__ blo
   const Block blk = blocks[b];
   const uint8_t* inb = in + blk.src;
   const uint32_t n = blk.raw_len;
-  const uint32_t* m = lazy + ((size_t)b << blog);
+  const uint32_t* m = match + ((size_t)b << blog);
+  if (blk.level) { uint32_t nc; level_params(blk.level, nc, lencap); }
   const uint32_t* mk = mark + ((size_t)b << (blog - 5));
   uint8_t* lit = T.lit + ((size_t)b << blog);
   uint32_t* LB = T.LB + (size_t)b * G.TCAP;
@@ -755,10 +751,17 @@ __global__ __launch_bounds__(64) void k_enc_tokens(const Block* __restrict__ blo
     }
     const uint32_t base = p0 + 8 * lane;
     const uint32_t bits = (cw >> (base & 31u)) & 0xFFu;  // which of this lane's 8 positions are on the chain
+    // the lazy values: the match value after the lane's last comes from the next lane, or the next step's lane 0
+    // (the shuffle runs with every lane active: a lane that sits out gives the lane before it nothing)
+    const uint32_t from_next = (uint32_t)__shfl_down((int)cm[0], 1, 64);
+    const uint32_t next_step = p0 + 512 < n ? readlane(nm[0], 0) : 0u;
+    const uint32_t after = lane == 63u ? next_step : from_next;
     uint32_t tmask = 0;
 #pragma unroll
-    for (uint32_t u = 0; u < 8; ++u)
+    for (uint32_t u = 0; u < 8; ++u) {
+      cm[u] = lazy_of(cm[u], base + u + 1u < n ? (u < 7 ? cm[u + 1] : after) : 0u, lencap);
       if (((bits >> u) & 1u) && match_at(cm[u], base + u, n)) tmask |= 1u << u;
+    }
     const uint32_t lmask = bits & ~tmask;
     const uint32_t lc = (uint32_t)__builtin_popcount(lmask), tc = (uint32_t)__builtin_popcount(tmask);
     const uint32_t li = wave_scan_incl(lc), ti = wave_scan_incl(tc);
