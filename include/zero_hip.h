@@ -123,6 +123,12 @@ int zgpu_decompress(zgpu_t* g, const uint8_t* comp, const uint32_t* comp_len, co
                     uint32_t* crc);
 /* crc[b] of the nb blocks of 2^block_log in data[0..n) (host buffer) */
 int zgpu_crc32(zgpu_t* g, const uint8_t* data, uint64_t n, uint32_t block_log, uint32_t* crc);
+/* crc[i] = CRC-32 of bytes [off[i], off[i] + len[i]) of the last zgpu_compress input (which = 0) or the last
+ * zgpu_decompress output (which = 1), which stay on the device until the next host-buffer call on g. For the
+ * container's per-file checksums of the solid stream, which the reference computes on the host while it reads and
+ * writes files (src/archive.h:279-287,320,707,774). ZGPU_ERR_ARG for a range outside that data, or when another
+ * host-buffer call came in between. */
+int zgpu_crc32_ranges(zgpu_t* g, int which, uint32_t n, const uint64_t* off, const uint64_t* len, uint32_t* crc);
 
 /* ---- the batched form: the blocks of many files, of uneven lengths, planned together (no reference counterpart:
  * the reference launches its stages per 16 MiB batch of full blocks, src/gpu_codec.h:153-229, and assumes every

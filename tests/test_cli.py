@@ -85,6 +85,20 @@ def test_gpu_archives_both_ways(tmp_path):
 
 
 @pytest.mark.gpu
+def test_gpu_file_checksums(tmp_path):
+    """files' CRCs come from the GPU: a table entry that disagrees with intact blocks is caught, in t and x"""
+    files = [("a", b"x" * 100000), ("b", bytes(range(256)) * 700), ("c", b""), ("d", b"tail")]
+    good = str(tmp_path / "good.zero")
+    write_archive(good, files, 16, 8)
+    assert run("t", good, "-q").returncode == 0
+    bad = str(tmp_path / "bad.zero")
+    write_archive(bad, files, 16, 8, file_crc={"b": 12345})
+    for args in (("t", bad, "-q"), ("x", bad, "-o", str(tmp_path / "out"), "-q")):
+        r = run(*args)
+        assert r.returncode == 1 and "b does not match its checksum" in r.stderr, (args, r.stderr)
+
+
+@pytest.mark.gpu
 def test_gpu_damaged_archive_reports(tmp_path):
     arc = str(tmp_path / "a.zero")
     write_archive(arc, [("f", (b"abcdefgh" * 30000) + bytes(range(256)) * 100)], 16, 32)

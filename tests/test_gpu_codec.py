@@ -73,6 +73,28 @@ def test_crc32(gpu):
             assert crc.tolist() == [zlib.crc32(b) for b in split(d, bl)], (bl, n)
 
 
+def test_crc32_ranges(gpu):
+    """CRCs of byte ranges (files' pieces) of the last compress input and decompress output, against zlib"""
+    rng = random.Random(3)
+    d = pyzero.gen_lz(4, (5 << 18) + 777).tobytes()
+    n = len(d)
+    cases = [(0, 0), (0, n), (n, 0), (0, 1), (n - 1, 1), (1 << 18, 1 << 18), (1000, (1 << 18) + 3), (5, (3 << 18) + 9)]
+    cases += [(o, rng.randrange(0, n - o + 1)) for o in (rng.randrange(n) for _ in range(40))]
+    got = gpu.compress(d, 18, 5)
+    want = [zlib.crc32(d[o: o + k]) for o, k in cases]
+    assert gpu.crc32_ranges(0, cases) == want
+    assert gpu.crc32_ranges(0, []) == []
+    with pytest.raises(pyzero.ZeroError):
+        gpu.crc32_ranges(0, [(n, 1)])
+    with pytest.raises(pyzero.ZeroError):
+        gpu.crc32_ranges(1, [(0, 1)])  # (no decompress output since the compress)
+    back = gpu.decompress([(m, len(b), p if m else b) for b, (m, p) in zip(split(d, 18), got)], 18)
+    assert back == d
+    assert gpu.crc32_ranges(1, cases) == want
+    with pytest.raises(pyzero.ZeroError):
+        gpu.crc32_ranges(0, [(0, 1)])  # (the decompress replaced the compress input)
+
+
 def test_damaged_payloads(gpu):
     """damage is refused or decodes like the sequential decoder; it never faults"""
     d = corpus(small=True)["mixed"][: 1 << 16]

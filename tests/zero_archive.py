@@ -10,8 +10,9 @@ from oracle_lib import Oracle
 O = Oracle()
 
 
-def write_archive(path, entries, block_log=18, depth=32):
-    """entries: [(name, bytes) for a file, (name, None) for a folder], in stream order"""
+def write_archive(path, entries, block_log=18, depth=32, file_crc=None):
+    """entries: [(name, bytes) for a file, (name, None) for a folder], in stream order; file_crc: {name: crc} to
+    store instead of the right CRC (a table that disagrees with intact blocks)"""
     B = 1 << block_log
     stream = b"".join(d for _, d in entries if d is not None)
     nblocks = (len(stream) + B - 1) // B
@@ -30,7 +31,8 @@ def write_archive(path, entries, block_log=18, depth=32):
     for name, d in entries:
         nb = name.encode()
         tbl += struct.pack("<B3xI", 1 if d is None else 0, len(nb)) + nb
-        tbl += struct.pack("<QqII", 0 if d is None else len(d), 1700000000, 0, 0 if d is None else zlib.crc32(d))
+        crc = 0 if d is None else (file_crc or {}).get(name, zlib.crc32(d))
+        tbl += struct.pack("<QqII", 0 if d is None else len(d), 1700000000, 0, crc)
     table_off = 24 + comp_total + 16 * nblocks
     assert table_off == len(out)
     out += tbl + struct.pack("<QQI", table_off, len(tbl), zlib.crc32(bytes(tbl))) + b"ZEND"

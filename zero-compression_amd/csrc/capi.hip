@@ -37,6 +37,10 @@ struct zgpu {
   zgpu_props_t props{};
   uint32_t* xpow = nullptr;  // x^(8 m 256^j) mod P, 3 x 256 (CRC combine)
   DBuf enc, dec, aux;  // encoder scratch, decoder scratch, staging of the host-buffer forms
+  DBuf rng;            // byte-range CRC tables
+  // the last zgpu_compress input [0] and zgpu_decompress output [1] still in aux (length 0: none)
+  const uint8_t* last_data[2] = {nullptr, nullptr};
+  uint64_t last_len[2] = {0, 0};
   bool profile = false;
   std::vector<Stage> stages;
   std::vector<hipEvent_t> pool;
@@ -241,7 +245,7 @@ void zgpu_close(zgpu_t* g) {
   if (!g) return;
   (void)hipSetDevice(g->dev);
   (void)hipStreamSynchronize(g->stream);
-  for (DBuf* b : {&g->enc, &g->dec, &g->aux})
+  for (DBuf* b : {&g->enc, &g->dec, &g->aux, &g->rng})
     if (b->p) (void)hipFree(b->p);
   for (hipEvent_t e : g->pool) (void)hipEventDestroy(e);
   if (g->xpow) (void)hipFree(g->xpow);
@@ -449,8 +453,11 @@ int zgpu_compress(zgpu_t* g, const uint8_t* in, uint64_t n, uint32_t blog, int l
     tab[b].level = 0;
   }
   const size_t outb = (size_t)nb * B;
+  g->last_len[0] = g->last_len[1] = 0;
   if (int rc = ensure(g, g->aux, al(n) + al(outb) + 2 * al((size_t)nb * 4) + al((size_t)nb * 32))) return rc;
   uint8_t* d_in = (uint8_t*)g->aux.p;
+  g->last_data[0] = d_in;
+  g->last_len[0] = n;
   uint8_t* d_out = d_in + al(n);
   uint32_t* d_len = (uint32_t*)(d_out + al(outb));
   uint32_t* d_crc = (uint32_t*)((uint8_t*)d_len + al((size_t)nb * 4));
@@ -497,9 +504,12 @@ int zgpu_decompress(zgpu_t* g, const uint8_t* comp, const uint32_t* comp_len, co
     co += comp_len[b];
     ro += raw_len[b];
   }
+  g->last_len[0] = g->last_len[1] = 0;
   if (int rc = ensure(g, g->aux, al(co) + al(ro) + 2 * al((size_t)nb * 4) + al((size_t)nb * 32))) return rc;
   uint8_t* d_comp = (uint8_t*)g->aux.p;
   uint8_t* d_out = d_comp + al(co);
+  g->last_data[1] = d_out;
+  g->last_len[1] = ro;
   uint32_t* d_err = (uint32_t*)(d_out + al(ro));
   uint32_t* d_crc = (uint32_t*)((uint8_t*)d_err + al((size_t)nb * 4));
   zgpu_block_t* d_tab = (zgpu_block_t*)((uint8_t*)d_crc + al((size_t)nb * 4));
@@ -533,6 +543,7 @@ int zgpu_crc32(zgpu_t* g, const uint8_t* data, uint64_t n, uint32_t blog, uint32
     tab[b].src = (uint64_t)b * B;
     tab[b].raw_len = (uint32_t)std::min<uint64_t>(B, n - (uint64_t)b * B);
   }
+  g->last_len[0] = g->last_len[1] = 0;
   if (int rc = ensure(g, g->aux, al(n) + al((size_t)nb * 4) + al((size_t)nb * 32))) return rc;
   uint8_t* d_in = (uint8_t*)g->aux.p;
   uint32_t* d_crc = (uint32_t*)(d_in + al(n));
@@ -542,6 +553,62 @@ int zgpu_crc32(zgpu_t* g, const uint8_t* data, uint64_t n, uint32_t blog, uint32
   if (int rc = zgpu_crc32_blocks(g, d_in, d_tab, nb, 0, d_crc, nullptr)) return rc;
   HIPCHK(g, hipMemcpyAsync(crc, d_crc, (size_t)nb * 4, hipMemcpyDeviceToHost, g->stream));
   HIPCHK(g, hipStreamSynchronize(g->stream));
+  return ZGPU_OK;
+}
+
+// ---------------------------------------------------------------- CRCs of byte ranges (files in the solid stream)
+namespace {
+uint32_t crc_mulmod(uint32_t a, uint32_t b) {  // reflected: bit 31 is x^0
+  uint32_t r = 0;
+  for (int i = 31; i >= 0; --i) {
+    if ((a >> i) & 1u) r ^= b;
+    b = (b >> 1) ^ ((b & 1u) ? 0xEDB88320u : 0u);
+  }
+  return r;
+}
+uint32_t crc_combine(uint32_t a, uint32_t b, uint64_t len_b) {  // crc(A||B) from crc(A), crc(B), |B|
+  uint32_t acc = 1u << 31, sq = 1u << 23;                      // x^0, x^8
+  for (uint64_t e = len_b; e; e >>= 1) {
+    if (e & 1u) acc = crc_mulmod(acc, sq);
+    sq = crc_mulmod(sq, sq);
+  }
+  return crc_mulmod(a, acc) ^ b;
+}
+constexpr uint64_t RANGE_PIECE = 1u << 18;  // ranges go to the GPU in pieces of at most this (one wavefront each)
+}  // namespace
+
+int zgpu_crc32_ranges(zgpu_t* g, int which, uint32_t n, const uint64_t* off, const uint64_t* len, uint32_t* crc) {
+  if (which != 0 && which != 1) return fail(g, ZGPU_ERR_ARG, "which: 0 (last compress input) or 1 (last decompress output)");
+  if (n == 0) return ZGPU_OK;
+  const uint64_t total = g->last_len[which];
+  std::vector<zgpu_block_t> tab;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (off[i] > total || len[i] > total - off[i])
+      return fail(g, ZGPU_ERR_ARG, "range outside the last call's data (or no such call since the last other one)");
+    for (uint64_t o = 0; o < len[i]; o += RANGE_PIECE) {
+      zgpu_block_t t{};
+      t.src = off[i] + o;
+      t.raw_len = (uint32_t)std::min<uint64_t>(RANGE_PIECE, len[i] - o);
+      tab.push_back(t);
+    }
+  }
+  const uint32_t np = (uint32_t)tab.size();
+  std::vector<uint32_t> pc(np);
+  if (np) {
+    if (int rc = ensure(g, g->rng, al((size_t)np * 32) + al((size_t)np * 4))) return rc;
+    zgpu_block_t* d_tab = (zgpu_block_t*)g->rng.p;
+    uint32_t* d_crc = (uint32_t*)((uint8_t*)g->rng.p + al((size_t)np * 32));
+    HIPCHK(g, hipMemcpyAsync(d_tab, tab.data(), (size_t)np * 32, hipMemcpyHostToDevice, g->stream));
+    if (int rc = zgpu_crc32_blocks(g, g->last_data[which], d_tab, np, 0, d_crc, nullptr)) return rc;
+    HIPCHK(g, hipMemcpyAsync(pc.data(), d_crc, (size_t)np * 4, hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(g, hipStreamSynchronize(g->stream));
+  }
+  uint32_t k = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    uint32_t c = 0;
+    for (uint64_t o = 0; o < len[i]; o += RANGE_PIECE, ++k) c = crc_combine(c, pc[k], tab[k].raw_len);
+    crc[i] = c;
+  }
   return ZGPU_OK;
 }
 

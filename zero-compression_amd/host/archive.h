@@ -151,13 +151,62 @@ struct CompressStats {
   double seconds = 0, codec_seconds = 0;
 };
 
-// the entries' bytes in order, in batches, with each file's CRC (src/archive.h:289-332)
+// Each file's CRC-32 from the CRCs of its pieces in the batches of the solid stream, which the codec computes where
+// the batch already is (on the GPU): crc(A||B) = combine(crc(A), crc(B), |B|). The reference checksums files on the
+// host as it reads and writes them (src/archive.h:279-287,320,707,774). finished(i, crc) is called for every file
+// once all its bytes have gone by, in entry order (empty files with CRC 0).
+struct FileCrcs {
+  const std::vector<Entry>& es;
+  size_t idx = 0;         // the entry the next byte of the stream belongs to
+  uint64_t in_file = 0;   // its bytes already seen
+  uint32_t crc = 0;       // their CRC
+  explicit FileCrcs(const std::vector<Entry>& e) : es(e) {}
+  template <class F> void settle(F&& finished) {
+    while (idx < es.size()) {
+      const Entry& e = es[idx];
+      if (e.dir) { ++idx; continue; }
+      if (in_file < e.size) return;
+      finished(idx, e.size ? crc : 0u);
+      ++idx;
+      in_file = 0;
+      crc = 0;
+    }
+  }
+  // the next n bytes of the stream, the last compress() input (input) or decompress() output of the codec
+  template <class F> void batch(Codec& codec, bool input, uint64_t n, F&& finished) {
+    std::vector<uint64_t> off, len;
+    uint64_t at = 0;
+    for (size_t i = idx, d = in_file; at < n;) {  // (the pieces, as the fold below will meet them)
+      if (i >= es.size()) throw Error("archive damaged: more data than the table lists");
+      const Entry& e = es[i];
+      if (e.dir || d >= e.size) { ++i; d = 0; continue; }
+      const uint64_t take = std::min<uint64_t>(n - at, e.size - d);
+      off.push_back(at);
+      len.push_back(take);
+      at += take;
+      d += take;
+    }
+    std::vector<uint32_t> pc;
+    if (!codec.range_crcs(input, off, len, pc)) throw Error("the codec lost its checksum support");
+    settle(finished);
+    for (size_t k = 0; k < off.size(); ++k) {
+      crc = crc::combine(crc, pc[k], len[k]);
+      in_file += len[k];
+      settle(finished);
+    }
+  }
+  template <class F> void close(F&& finished) { settle(finished); }
+};
+
+// the entries' bytes in order, in batches, with each file's CRC (src/archive.h:289-332) unless the codec computes
+// those (host_crc false)
 struct SolidReader {
   std::vector<Entry>& es;
   size_t idx = 0;
   uint64_t pos = 0;
   FILE* f = nullptr;
   uint32_t crc = 0;
+  bool host_crc = true;
   explicit SolidReader(std::vector<Entry>& e) : es(e) {}
   ~SolidReader() { if (f) fclose(f); }
   size_t read(uint8_t* buf, size_t n) {
@@ -165,7 +214,7 @@ struct SolidReader {
     while (got < n && idx < es.size()) {
       Entry& e = es[idx];
       if (e.dir || pos >= e.size) {
-        if (!e.dir) e.crc = crc;
+        if (!e.dir && host_crc) e.crc = crc;
         if (f) { fclose(f); f = nullptr; }
         crc = 0;
         ++idx;
@@ -176,14 +225,14 @@ struct SolidReader {
       const size_t want = (size_t)std::min<uint64_t>(n - got, e.size - pos);
       const size_t r = fread(buf + got, 1, want, f);
       if (r != want) memset(buf + got + r, 0, want - r);  // (a file that shrank: zeros keep the table right)
-      crc = crc::update(crc, buf + got, want);
+      if (host_crc) crc = crc::update(crc, buf + got, want);
       got += want;
       pos += want;
     }
     return got;
   }
   void finish() {
-    if (idx < es.size() && !es[idx].dir && pos >= es[idx].size) es[idx].crc = crc;
+    if (host_crc && idx < es.size() && !es[idx].dir && pos >= es[idx].size) es[idx].crc = crc;
     if (f) { fclose(f); f = nullptr; }
     idx = es.size();
   }
@@ -215,6 +264,11 @@ inline CompressStats compress_archive(const std::vector<std::string>& inputs, co
   size_t batch = std::max(B, codec.batch_bytes(block_log) / B * B);
   std::vector<uint8_t> buf(batch), next_buf(batch);
   SolidReader reader(es);
+  FileCrcs files(es);
+  std::vector<uint32_t> probe;
+  const bool codec_crc = codec.range_crcs(true, {}, {}, probe);  // (the files' CRCs on the GPU when it can)
+  reader.host_crc = !codec_crc;
+  auto set_crc = [&es](size_t i, uint32_t c) { es[i].crc = c; };
   uint64_t done = 0, queued = 0;
   std::vector<BlockOut> outs;
   rep.at(0, total);
@@ -230,6 +284,7 @@ inline CompressStats compress_archive(const std::vector<std::string>& inputs, co
     if (queued < total) ahead = std::async(std::launch::async, fill, std::ref(next_buf), total - queued);
     const double c0 = now();
     codec.compress(buf.data(), n, block_log, outs);
+    if (codec_crc) files.batch(codec, true, n, set_crc);
     st.codec_seconds += now() - c0;
     const uint32_t* crcs = codec.block_crcs();
     if (!crcs) {
@@ -262,6 +317,7 @@ inline CompressStats compress_archive(const std::vector<std::string>& inputs, co
     }
   }
   reader.finish();
+  if (codec_crc) files.close(set_crc);
   std::vector<uint8_t> tbl;
   put32(tbl, (uint32_t)es.size());
   for (const Entry& e : es) {
@@ -291,6 +347,7 @@ struct ArchiveInfo {
   uint32_t block_log = 0;
   uint64_t total_raw = 0, n_blocks = 0, table_off = 0, file_size = 0;
   std::vector<Entry> entries;
+  bool codec_file_crcs = false;  // read_blocks checks the files' CRCs with the codec: sinks need not
 };
 
 // header, footer and entry table, every field checked (src/archive.h:452-501)
@@ -383,6 +440,13 @@ inline ExtractStats read_blocks(const std::string& path, Codec& codec, Report& r
   struct Closer { FILE* f; ~Closer() { fclose(f); } } c{f};
   a = read_info(f);
   codec.prepare(a.block_log);
+  std::vector<uint32_t> probe;
+  a.codec_file_crcs = codec.range_crcs(false, {}, {}, probe);  // (the files' CRCs on the GPU when it can)
+  FileCrcs files(a.entries);
+  auto check_crc = [&a](size_t i, uint32_t c) {
+    if (a.entries[i].size > 0 && c != a.entries[i].crc)
+      throw Error("file " + a.entries[i].name + " does not match its checksum");
+  };
   fseeko(f, 24, SEEK_SET);
   const size_t B = (size_t)1 << a.block_log;
   const size_t per_batch = std::max(B, codec.batch_bytes_decode(a.block_log) / B * B) / B;
@@ -451,6 +515,7 @@ inline ExtractStats read_blocks(const std::string& path, Codec& codec, Report& r
     for (size_t b = 0; b < nb; ++b)
       if (got[b] != cur.crcs[b])
         throw Error("block " + std::to_string(st.blocks + b) + " does not match its checksum: the archive is damaged");
+    if (a.codec_file_crcs) files.batch(codec, false, cur.raw_total, check_crc);
     const size_t raw_total = cur.raw_total;
     writing = std::async(std::launch::async, [&, raw, raw_total, nb] {
       sink(raw, raw_total);
@@ -463,6 +528,7 @@ inline ExtractStats read_blocks(const std::string& path, Codec& codec, Report& r
   }
   if (writing.valid()) writing.get();
   if (done != a.total_raw) throw Error("archive damaged: the blocks hold " + std::to_string(done) + " bytes, the table " + std::to_string(a.total_raw));
+  if (a.codec_file_crcs) files.close(check_crc);
   st.raw = done;
   st.seconds = now() - t0;
   return st;
@@ -476,6 +542,7 @@ struct SolidWriter {
   uint64_t pos = 0;
   FILE* f = nullptr;
   uint32_t crc = 0;
+  bool host_crc = true;  // (false: read_blocks checks the files' CRCs)
   size_t files = 0, folders = 0;
   SolidWriter(const std::vector<Entry>& e, const fs::path& r) : es(e), root(r) {}
   ~SolidWriter() { if (f) fclose(f); }
@@ -491,7 +558,7 @@ struct SolidWriter {
   void finish_file() {
     const Entry& e = es[idx];
     if (f) { fclose(f); f = nullptr; }
-    if (!e.dir && crc != e.crc && e.size > 0) throw Error("file " + e.name + " does not match its checksum");
+    if (host_crc && !e.dir && crc != e.crc && e.size > 0) throw Error("file " + e.name + " does not match its checksum");
     set_mtime(root / fs::path(e.name), e.mtime);
     crc = 0;
   }
@@ -519,7 +586,7 @@ struct SolidWriter {
       if (!f) open_next();
       const size_t want = (size_t)std::min<uint64_t>(n, e.size - pos);
       if (fwrite(p, 1, want, f) != want) throw Error("writing " + e.name + " failed (disk full?)");
-      crc = crc::update(crc, p, want);
+      if (host_crc) crc = crc::update(crc, p, want);
       p += want;
       n -= want;
       pos += want;
@@ -540,7 +607,7 @@ inline ExtractStats extract_archive(const std::string& path, const std::string& 
   fs::create_directories(root, ec);
   std::unique_ptr<SolidWriter> w;
   ExtractStats st = read_blocks(path, codec, rep, a, [&](const uint8_t* p, size_t n) {
-    if (!w) w = std::make_unique<SolidWriter>(a.entries, root);
+    if (!w) { w = std::make_unique<SolidWriter>(a.entries, root); w->host_crc = !a.codec_file_crcs; }
     w->write(p, n);
   });
   if (!w) w = std::make_unique<SolidWriter>(a.entries, root);
@@ -561,7 +628,7 @@ inline ExtractStats test_archive(const std::string& path, Codec& codec, Report& 
       const Entry& e = a.entries[idx];
       if (e.dir) { ++idx; continue; }
       if (pos >= e.size) {
-        if (e.size > 0 && crc != e.crc) throw Error("file " + e.name + " does not match its checksum");
+        if (!a.codec_file_crcs && e.size > 0 && crc != e.crc) throw Error("file " + e.name + " does not match its checksum");
         ++files;
         ++idx;
         pos = 0;
@@ -577,7 +644,7 @@ inline ExtractStats test_archive(const std::string& path, Codec& codec, Report& 
       if (idx >= a.entries.size()) throw Error("archive damaged: more data than the table lists");
       const Entry& e = a.entries[idx];
       const size_t want = (size_t)std::min<uint64_t>(n, e.size - pos);
-      crc = crc::update(crc, p, want);
+      if (!a.codec_file_crcs) crc = crc::update(crc, p, want);
       p += want;
       n -= want;
       pos += want;

@@ -3,8 +3,9 @@
 // The interface keeps the reference's shape (src/archive.h:45-69): compress() turns a batch of whole blocks into one
 // BlockOut per block, decompress() turns a list of BlockIn into their raw bytes back to back, in memory the codec owns
 // and keeps valid through the next call (the container checks and writes batch k-1 while batch k decodes). The one
-// addition is block_crcs(): the CRC-32 of every block of the last batch, computed on the GPU, so the container need
-// not checksum blocks on the host (src/archive.h:376,618).
+// additions are block_crcs() and range_crcs(): the CRC-32 of every block of the last batch, and of any byte ranges
+// of it (the files' pieces), computed on the GPU where the batch already is, so the container need not checksum
+// blocks or files on the host (src/archive.h:279-287,376,618).
 //
 // GpuCodec replaces src/gpu_codec.h: it calls libzero_hip.so (include/zero_hip.h), whose kernels are compiled ahead of
 // time for gfx950, so prepare() has nothing to build.
@@ -51,6 +52,13 @@ struct Codec {
   virtual void prepare(uint32_t) {}
   // CRC-32 of each block of the last compress() input or decompress() output; nullptr when not computed
   virtual const uint32_t* block_crcs() const { return nullptr; }
+  // CRC-32 of byte ranges [off[i], off[i] + len[i]) of the last compress() input (input) or decompress() output,
+  // computed where that data already is; false when the codec cannot (the container then checksums files itself)
+  virtual bool range_crcs(bool input, const std::vector<uint64_t>& off, const std::vector<uint64_t>& len,
+                          std::vector<uint32_t>& crc) {
+    (void)input; (void)off; (void)len; (void)crc;
+    return false;
+  }
 };
 
 class GpuCodec : public Codec {
@@ -77,6 +85,13 @@ class GpuCodec : public Codec {
   size_t batch_bytes_decode(uint32_t) const override { return (size_t)1 << 30; }
   void prepare(uint32_t block_log) override { block_log_ = block_log; }
   const uint32_t* block_crcs() const override { return crcs_.empty() ? nullptr : crcs_.data(); }
+  // on the GPU, from the batch the last call left there (zgpu_crc32_ranges)
+  bool range_crcs(bool input, const std::vector<uint64_t>& off, const std::vector<uint64_t>& len,
+                  std::vector<uint32_t>& crc) override {
+    crc.assign(off.size(), 0);
+    check(zgpu_crc32_ranges(g_, input ? 0 : 1, (uint32_t)off.size(), off.data(), len.data(), crc.data()));
+    return true;
+  }
 
   void compress(const uint8_t* in, size_t n, uint32_t block_log, std::vector<BlockOut>& out) override {
     const size_t B = (size_t)1 << block_log;

@@ -31,7 +31,7 @@ EXPORTS = (
     "zgpu_event_create", "zgpu_event_destroy", "zgpu_event_record", "zgpu_event_elapsed_ms",
     "zgpu_encode_blocks", "zgpu_decode_blocks", "zgpu_crc32_blocks",
     "zgpu_encode_scratch_bytes", "zgpu_decode_scratch_bytes",
-    "zgpu_compress", "zgpu_decompress", "zgpu_crc32",
+    "zgpu_compress", "zgpu_decompress", "zgpu_crc32", "zgpu_crc32_ranges",
     "zgpu_set_profile", "zgpu_stage_times",
     "zgpu_plan_create", "zgpu_plan_destroy", "zgpu_plan_slab_bytes", "zgpu_plan_slab_offsets", "zgpu_plan_classes",
     "zgpu_plan_encode", "zgpu_plan_set_stored", "zgpu_plan_decode", "zgpu_plan_tune", "zgpu_plan_use_graphs",
@@ -110,6 +110,7 @@ def load_library(path=LIB_PATH):
         "zgpu_compress": (i32, [vp, vp, u64, u32, i32, vp, vp, vp]),
         "zgpu_decompress": (i32, [vp, vp, vp, vp, vp, u32, u32, vp, vp, vp]),
         "zgpu_crc32": (i32, [vp, vp, u64, u32, vp]),
+        "zgpu_crc32_ranges": (i32, [vp, i32, u32, vp, vp, vp]),
         "zgpu_set_profile": (i32, [vp, i32]),
         "zgpu_stage_times": (i32, [vp, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_float), i32]),
     }
@@ -254,6 +255,14 @@ class Gpu:
         if nb:
             self._check(self.lib.zgpu_crc32(self.h, _ptr(a), a.size, block_log, _ptr(crc)))
         return crc[:nb]
+
+    def crc32_ranges(self, which, ranges):
+        """CRC-32 of (offset, length) ranges of the last compress input (which 0) or decompress output (which 1)."""
+        off = np.array([r[0] for r in ranges], dtype=np.uint64)
+        ln = np.array([r[1] for r in ranges], dtype=np.uint64)
+        crc = np.zeros(max(1, len(ranges)), dtype=np.uint32)
+        self._check(self.lib.zgpu_crc32_ranges(self.h, int(which), len(ranges), _ptr(off), _ptr(ln), _ptr(crc)))
+        return [int(c) for c in crc[: len(ranges)]]
 
     # ---- device-resident forms
     def alloc(self, nbytes):
