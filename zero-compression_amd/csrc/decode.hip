@@ -102,23 +102,33 @@ __device__ inline void shift16(uint32_t& v0, uint32_t& v1, uint32_t& v2, uint32_
 }
 
 // one rANS decoding step of a lane: the symbol of state x (slot table, then its freq | cum << 16) goes to *out; the
-// renormalisation takes 0, 1 or 2 bytes from the window c0..c3 at byte offset o (at most 14), and o advances
-__device__ inline void rans_step(const uint8_t* slot, const uint32_t* fc, uint8_t* out, bool valid, uint32_t c0,
-                                 uint32_t c1, uint32_t c2, uint32_t c3, uint32_t& x, uint32_t& o) {
+// renormalisation takes 0, 1 or 2 bytes from the front of the lane's next bytes blo:bhi (first byte lowest), which
+// then move on by as many, and o counts them
+__device__ inline void rans_step(const uint8_t* slot, const uint32_t* fc, uint8_t* out, bool valid, uint32_t& blo,
+                                 uint32_t& bhi, uint32_t& x, uint32_t& o) {
   const uint32_t sl = x & 4095u;
   const uint32_t sy = slot[sl];
   const uint32_t e = fc[sy];
   *out = (uint8_t)sy;
   const uint32_t xs = (e & 0xFFFFu) * (x >> 12) + sl - (e >> 16);
-  const uint32_t need = (xs < (1u << 15) ? 1u : 0u) + (xs < (1u << 23) ? 1u : 0u);
-  const uint32_t q = o >> 2;
-  const uint32_t lo = q == 0 ? c0 : q == 1 ? c1 : q == 2 ? c2 : c3;
-  const uint32_t hi = q == 0 ? c1 : q == 1 ? c2 : q == 2 ? c3 : 0u;
-  const uint32_t w2 = __builtin_amdgcn_alignbyte(hi, lo, o & 3u);
-  const uint32_t v = (w2 & 0xFFu) << 8 | ((w2 >> 8) & 0xFFu);  // first byte high
-  const uint32_t xn = (xs << (8u * need)) | (v >> (16u - 8u * need));
+  const uint32_t need = valid ? (xs < (1u << 15) ? 1u : 0u) + (xs < (1u << 23) ? 1u : 0u) : 0u;
+  const uint32_t s8 = 8u * need;
+  const uint32_t t16 = ((blo & 0xFFu) << 8) | ((blo >> 8) & 0xFFu);  // first byte high
+  const uint32_t xn = (xs << s8) | (t16 >> (16u - s8));
   x = valid ? xn : x;
-  o += valid ? need : 0u;
+  blo = __builtin_amdgcn_alignbyte(bhi, blo, need);
+  bhi >>= s8;
+  o += need;
+}
+// bytes [o, o + 8) of the 16-byte window c0..c3 (o <= 8) as blo:bhi
+__device__ inline void window8(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t o, uint32_t& blo,
+                               uint32_t& bhi) {
+  const uint32_t q = o >> 2, r = o & 3u;
+  const uint32_t w0 = q == 0 ? c0 : q == 1 ? c1 : c2;
+  const uint32_t w1 = q == 0 ? c1 : q == 1 ? c2 : c3;
+  const uint32_t w2 = q == 0 ? c2 : q == 1 ? c3 : 0u;  // (q = 2 comes with r = 0)
+  blo = __builtin_amdgcn_alignbyte(w1, w0, r);
+  bhi = __builtin_amdgcn_alignbyte(w2, w1, r);
 }
 
 // ---------------------------------------------------------------- the streams: one wavefront per (block, stream)
@@ -223,9 +233,13 @@ __global__ __launch_bounds__(64) void k_dec_streams(const Block* __restrict__ bl
   const uint32_t nsteps = (n + 63) >> 6, nfull = n >> 6;  // (every lane has a symbol in the first nfull steps)
   uint32_t b0 = 0;
   for (; b0 + 8 <= nfull; b0 += 8) {
-    uint32_t o = 0;
+    // 4 steps from the window's first 8 bytes (they take at most 8), 4 from the 8 after what those took
+    uint32_t o = 0, blo = c0, bhi = c1;
 #pragma unroll
-    for (uint32_t k = 0; k < 8; ++k) rans_step(slot, fc, stg + k * 64 + lane, true, c0, c1, c2, c3, x, o);
+    for (uint32_t k = 0; k < 4; ++k) rans_step(slot, fc, stg + k * 64 + lane, true, blo, bhi, x, o);
+    window8(c0, c1, c2, c3, o, blo, bhi);
+#pragma unroll
+    for (uint32_t k = 4; k < 8; ++k) rans_step(slot, fc, stg + k * 64 + lane, true, blo, bhi, x, o);
     // the window moves on by o bytes: bytes [o, o + 16) of c0..c3 then n0..n3 (which arrived meanwhile)
     shift16(n0, n1, n2, n3, nd);
     const uint32_t q = o >> 2, r = o & 3u;
@@ -246,9 +260,14 @@ __global__ __launch_bounds__(64) void k_dec_streams(const Block* __restrict__ bl
     wave_fence();
   }
   if (b0 < nsteps) {  // the last (at most 8) steps, within the window
-    uint32_t o = 0;
-    for (uint32_t k = 0; b0 + k < nsteps; ++k)
-      rans_step(slot, fc, stg + k * 64 + lane, (b0 + k) * 64 + lane < n, c0, c1, c2, c3, x, o);
+    uint32_t o = 0, blo = c0, bhi = c1;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k)
+      rans_step(slot, fc, stg + k * 64 + lane, (b0 + k) * 64 + lane < n, blo, bhi, x, o);
+    window8(c0, c1, c2, c3, o, blo, bhi);
+#pragma unroll
+    for (uint32_t k = 4; k < 8; ++k)
+      rans_step(slot, fc, stg + k * 64 + lane, (b0 + k) * 64 + lane < n, blo, bhi, x, o);
     pos += o;
     wave_fence();
     wave_copy_lds(dest + b0 * 64, stg, n - b0 * 64);
