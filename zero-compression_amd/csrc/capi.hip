@@ -138,8 +138,8 @@ EncLayout enc_layout(uint32_t nb, uint32_t blog, uint32_t hbits) {
   EncLayout L;
   size_t o = 0;
   L.qcap = (uint32_t)((size_t)nb * (G.B / 17u + 1u));
-  L.srt = o; o += al(N * 4);                // sorted bucket entries, then the lazy matches
-  L.mtch = o; o += al(N * 4);               // radix temporary, then the matches
+  L.srt = o; o += al(N * 4);                // bucketed positions, then sorted (mtch must follow: see k_enc_sort_s)
+  L.mtch = o; o += al(N * 4);               // positions by partition, radix temporary, then the matches
   L.bstart = o; o += al((size_t)nb * (HB + 1) * 4);
   L.qmid = o; o += al((size_t)L.qcap * 8);
   L.qlarge = o; o += al((size_t)L.qcap * 8);

@@ -236,67 +236,79 @@ struct SortQueues {
   uint32_t cap;     // pairs each queue can hold
 };
 
-// Buckets of <= 16 entries: one thread each, a bitonic network in registers. Buckets of 17..64: the thread's
+// Buckets of <= 16 entries: one thread each, a sorting network in registers. Buckets of 17..64: the thread's
 // wavefront sorts them one after another, one entry per lane, a bitonic network over lane shuffles. Larger buckets
 // are queued for k_enc_sort_m (<= 4096) or k_enc_sort_l.
+// the k <= 16 values at s sorted (one thread): Batcher's odd-even merge sort of 16, 63 comparators, each a min and a
+// max (the padding sorts last). All 16 reads are made: past the bucket they read entries that are not written.
+__device__ inline void sort16(uint32_t* s, uint32_t k) {
+  uint32_t v[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) v[i] = s[i];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) v[i] = (uint32_t)i < k ? v[i] : 0xFFFFFFFFu;
+#pragma unroll
+  for (int p = 1; p < 16; p <<= 1)
+#pragma unroll
+    for (int kk = p; kk >= 1; kk >>= 1)
+#pragma unroll
+      for (int j = kk % p; j + kk < 16; j += 2 * kk)
+#pragma unroll
+        for (int i = 0; i < kk && i + j + kk < 16; ++i)
+          if ((i + j) / (2 * p) == (i + j + kk) / (2 * p)) {
+            const uint32_t a = v[i + j], c = v[i + j + kk];
+            v[i + j] = min(a, c);
+            v[i + j + kk] = max(a, c);
+          }
+#pragma unroll
+  for (int i = 0; i < 16; ++i)
+    if ((uint32_t)i < k) s[i] = v[i];
+}
+// the k (17..64) values at s sorted by the whole wavefront, one per lane: a bitonic network over lane shuffles
+__device__ inline void sort64_wave(uint32_t* s, uint32_t k) {
+  const uint32_t lane = lane_id();
+  uint32_t v = lane < k ? s[lane] : 0xFFFFFFFFu;
+#pragma unroll
+  for (uint32_t kk = 2; kk <= 64; kk <<= 1)
+#pragma unroll
+    for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
+      const uint32_t o = (uint32_t)__shfl_xor((int)v, (int)j, 64);
+      const bool keep_min = ((lane & j) == 0) == ((lane & kk) == 0);
+      v = keep_min ? min(v, o) : max(v, o);
+    }
+  if (lane < k) s[lane] = v;
+}
+// the bucket [lo, lo + k) of each thread, at base + (lo - at)
+__device__ inline void sort_small(uint32_t* base, uint32_t at, uint32_t lo, uint32_t k, uint32_t b, uint32_t h,
+                                  SortQueues q) {
+  if (k > 64) {
+    const uint32_t which = k > 4096 ? 1 : 0;
+    const uint32_t i = atomicAdd(&q.counts[which], 1u);
+    if (i < q.cap) { uint32_t* dst = which ? q.large : q.mid; dst[2 * i] = b; dst[2 * i + 1] = h; }
+  } else if (k >= 2 && k <= 16) {
+    sort16(base + (lo - at), k);
+  }
+  uint64_t m = ballot(k > 16 && k <= 64);
+  while (m) {
+    const uint32_t who = (uint32_t)__builtin_ctzll(m);
+    m &= m - 1;
+    sort64_wave(base + (readlane(lo, who) - at), readlane(k, who));
+  }
+}
+
 __global__ __launch_bounds__(256) void k_enc_sort_s(const Block* __restrict__ blocks, uint32_t nb, uint32_t blog,
                                                     uint32_t hbits, uint32_t* __restrict__ srt,
                                                     const uint32_t* __restrict__ bstart, SortQueues q) {
   const uint32_t HB = 1u << hbits;
   const uint64_t gid = (uint64_t)xcd_swizzle(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   const uint32_t b = (uint32_t)(gid / HB), h = (uint32_t)(gid % HB);
-  const bool valid = b < nb;
   uint32_t lo = 0, k = 0;
-  if (valid) {
+  if (b < nb) {
     const uint32_t* bs = bstart + (size_t)b * (HB + 1);
     lo = bs[h];
     k = bs[h + 1] - lo;
   }
-  uint32_t* s = srt + ((size_t)b << blog) + lo;
-  if (k > 64) {
-    const uint32_t which = k > 4096 ? 1 : 0;
-    const uint32_t at = atomicAdd(&q.counts[which], 1u);
-    if (at < q.cap) { uint32_t* dst = which ? q.large : q.mid; dst[2 * at] = b; dst[2 * at + 1] = h; }
-  } else if (k >= 2 && k <= 16) {
-    uint32_t v[16];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) v[i] = (uint32_t)i < k ? s[i] : 0xFFFFFFFFu;
-#pragma unroll
-    for (int kk = 2; kk <= 16; kk <<= 1)
-#pragma unroll
-      for (int j = kk >> 1; j > 0; j >>= 1)
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-          const int l = i ^ j;
-          if (l > i) {
-            const bool asc = (i & kk) == 0;
-            const uint32_t a = v[i], c = v[l];
-            if ((a > c) == asc) { v[i] = c; v[l] = a; }
-          }
-        }
-#pragma unroll
-    for (int i = 0; i < 16; ++i)
-      if ((uint32_t)i < k) s[i] = v[i];
-  }
-  uint64_t m = ballot(k > 16 && k <= 64);
-  const uint32_t lane = lane_id();
-  while (m) {
-    const uint32_t who = (uint32_t)__builtin_ctzll(m);
-    m &= m - 1;
-    const uint32_t kw = readlane(k, who);
-    uint32_t* sw = (uint32_t*)(((uint64_t)readlane((uint32_t)((uint64_t)s >> 32), who) << 32) |
-                               readlane((uint32_t)(uint64_t)s, who));
-    uint32_t v = lane < kw ? sw[lane] : 0xFFFFFFFFu;
-#pragma unroll
-    for (uint32_t kk = 2; kk <= 64; kk <<= 1)
-#pragma unroll
-      for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
-        const uint32_t o = (uint32_t)__shfl_xor((int)v, (int)j, 64);
-        const bool keep_min = ((lane & j) == 0) == ((lane & kk) == 0);
-        v = keep_min ? min(v, o) : max(v, o);
-      }
-    if (lane < kw) sw[lane] = v;
-  }
+  sort_small(srt + ((size_t)b << blog), 0, lo, k, b, h, q);
 }
 
 // queued buckets of 65..4096 entries: one workgroup per bucket, bitonic sort in LDS (padded to a power of two)
