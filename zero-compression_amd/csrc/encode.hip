@@ -494,18 +494,21 @@ __device__ inline void match_slots(const uint8_t* __restrict__ inb, uint32_t n, 
     best = min(common_prefix<NW>(w, r), maxlen);
     bestd = 1;
   }
+  // every candidate compared, with selects rather than branches (lanes differ in how many share their key)
   bool go = act;
-  for (uint32_t k = 1; k <= ncand; ++k) {
+#pragma unroll
+  for (uint32_t k = 1; k <= MX; ++k) {
+    if (k > ncand) break;  // (uniform)
     const uint32_t idx = MX + lane - k;
     const uint32_t cp = lpos[idx], ck = lkey[idx];
     go = go && cp != NOPOS && ck == key;  // (sorted by key: no earlier slot has this key)
-    if (go && best < maxlen) {
-      uint32_t c[NW];
+    uint32_t c[NW];
 #pragma unroll
-      for (int q = 0; q < NW; ++q) c[q] = lwin[idx * NW + q];
-      const uint32_t l = min(common_prefix<NW>(w, c), maxlen);
-      if (l >= 4 && l > best) { best = l; bestd = p - cp; }
-    }
+    for (int q = 0; q < NW; ++q) c[q] = lwin[idx * NW + q];
+    const uint32_t l = min(common_prefix<NW>(w, c), maxlen);
+    const bool better = go && l >= 4 && l > best;
+    best = better ? l : best;
+    bestd = better ? p - cp : bestd;
   }
   if (act) mb[p] = best >= 4 ? (best << 20) | (bestd - 1) : 0u;
   else if (i >= count && i < n) mb[i] = 0;  // (the last 3 positions are never in srt: exactly these slots)
