@@ -123,7 +123,13 @@ struct EncLayout {
 struct EncShape {
   uint32_t blog, hbits, keybits, walkers;
 };
-uint32_t walkers_of(uint32_t blog) { return std::min<uint32_t>(PARSE_WALKERS, (1u << blog) / 32); }
+// the parse's segments per block: at most PARSE_WALKERS, each at least 32 positions, and the parse's LDS (the
+// chain bitmap, 3 words and a 16-word window per walker) within 160 KiB
+uint32_t walkers_of(uint32_t blog) {
+  uint32_t w = std::min<uint32_t>(PARSE_WALKERS, (1u << blog) / 32);
+  while (w > 1 && ((size_t)(1u << blog) / 32 + 19 * (size_t)w) * 4 > 160 * 1024) w >>= 1;
+  return w;
+}
 EncShape default_shape(uint32_t blog) { return {blog, hbits_of(blog), hbits_of(blog) + KEY_TAG_BITS, walkers_of(blog)}; }
 bool shape_ok(const EncShape& S) {
   return S.blog >= 10 && S.blog <= 20 && S.hbits >= 4 && S.hbits <= 15 && S.hbits <= S.keybits &&
