@@ -147,38 +147,37 @@ __global__ __launch_bounds__(1024) void k_enc_hash(const Block* __restrict__ blo
   }
   if (tid == 0) bs[HB] = count;
   __syncthreads();
-  // the positions grouped by partition (in no set order inside one: k_enc_place does not need it): for each of its 4
-  // positions, a wavefront ranks its lanes by partition from ballots over the partition's bits, and one lane per
-  // partition present claims that many slots with an LDS atomic (distinct addresses: no conflicts). No workgroup
-  // barrier inside the loop.
+  // the positions grouped by partition (in no set order inside one: k_enc_place does not need it): a wavefront counts
+  // its 256 positions per partition in LDS counters of its own (the atomics' return values rank the lanes), claims
+  // each partition's count once from the shared cursors, and the lanes place their positions after the claims. No
+  // workgroup barrier inside the loop.
   const uint32_t NP = 1u << pbits;
   uint32_t* pcur = cnt + HB + 16;       // [NP] next free slot of each partition
-  uint32_t* wb = pcur + NP + wv * NP;   // [16][NP] this wavefront's claims
+  uint32_t* wc = pcur + NP + wv * NP;   // [16][NP] this wavefront's counts, then its claims
   if (tid < NP) pcur[tid] = cnt[tid << (HBITS - pbits)];
   __syncthreads();
   uint32_t* tb = tmp + ((size_t)b << blog);
   uint64_t x = n >= 8 ? ld8z(inb, n, 4 * tid) : ld8z_small(inb, n, 4 * tid);
   for (uint32_t q = 4 * tid; q - 4 * lane < count; q += 4096) {  // (a wavefront runs while any lane has work)
     const uint64_t nx = n >= 8 ? ld8z(inb, n, q + 4096) : 0ull;
-    uint32_t slot[4];
+    for (uint32_t i = lane; i < NP; i += 64) wc[i] = 0;
+    wave_fence();
+    uint32_t d[4], rank[4];
 #pragma unroll
     for (uint32_t j = 0; j < 4; ++j) {
-      const bool valid = q + j < count;
-      const uint32_t d = valid && pbits ? hash5_of(x, j) >> (32 - pbits) : 0u;
-      uint64_t m = ballot(valid);
-      for (uint32_t bit = 0; bit < pbits; ++bit) {
-        const uint64_t bb = ballot((d >> bit) & 1u);
-        m &= ((d >> bit) & 1u) ? bb : ~bb;
-      }
-      const uint32_t rank = popc64(m & lanemask_lt());
-      if (valid && rank == 0) wb[d] = atomicAdd(&pcur[d], popc64(m));
-      wave_fence();
-      slot[j] = wb[d] + rank;
-      wave_fence();  // (the next claims overwrite wb)
+      d[j] = pbits ? hash5_of(x, j) >> (32 - pbits) : 0u;
+      rank[j] = q + j < count ? atomicAdd(&wc[d[j]], 1u) : 0u;
     }
+    wave_fence();
+    for (uint32_t i = lane; i < NP; i += 64) {
+      const uint32_t c = wc[i];
+      wc[i] = c ? atomicAdd(&pcur[i], c) : 0u;
+    }
+    wave_fence();
 #pragma unroll
     for (uint32_t j = 0; j < 4; ++j)
-      if (q + j < count) tb[slot[j]] = q + j;
+      if (q + j < count) tb[wc[d[j]] + rank[j]] = q + j;
+    wave_fence();  // (the next step's counts overwrite wc)
     x = nx;
   }
 }
