@@ -657,9 +657,11 @@ __global__ __launch_bounds__(64) void k_dec_lz(const Block* __restrict__ blocks,
     const uint32_t ngc = readlane(rg, c - 1) + 1;
     if (on && (uint64_t)xq + nx > ne) bad = true;
     uint32_t L = tv >> 4, M = on ? (tv & 15u) + 4u : 0u;
-    if (!bad) {
-      if (fl) L += (uint32_t)extb[xq] | (uint32_t)extb[xq + 1] << 8 | (uint32_t)extb[xq + 2] << 16;
-      if (fm) { const uint32_t q = xq + 3u * fl; M += (uint32_t)extb[q] | (uint32_t)extb[q + 1] << 8; }
+    if (!bad && (fl | fm)) {  // the token's (at most 5) ext bytes in one load (the decoder scratch goes on after them)
+      uint64_t w;
+      __builtin_memcpy(&w, extb + xq, 8);
+      if (fl) L += (uint32_t)w & 0xFFFFFFu;
+      if (fm) M += (uint32_t)(w >> (24u * fl)) & 0xFFFFu;
     }
     uint32_t D = 0;
     if (on && !bad && !rep) {
@@ -668,9 +670,16 @@ __global__ __launch_bounds__(64) void k_dec_lz(const Block* __restrict__ blocks,
       } else {
         uint32_t w = 0;
         const uint32_t at = bq >> 3;
+        if (nob >= 4) {  // 4 bytes at a clamped address, shifted: bytes past the section read as 0
+          const uint32_t a = min(at, nob - 4u);
+          uint32_t v;
+          __builtin_memcpy(&v, bits + a, 4);
+          w = at - a < 4u ? v >> (8u * (at - a)) : 0u;
+        } else {
 #pragma unroll
-        for (uint32_t k = 0; k < 4; ++k)
-          if (at + k < nob) w |= (uint32_t)bits[at + k] << (8 * k);
+          for (uint32_t k = 0; k < 4; ++k)
+            if (at + k < nob) w |= (uint32_t)bits[at + k] << (8 * k);
+        }
         D = (1u << nb_) + ((w >> (bq & 7u)) & ((1u << nb_) - 1u));
       }
     }
