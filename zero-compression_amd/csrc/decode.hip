@@ -635,11 +635,13 @@ __global__ __launch_bounds__(64) void k_dec_lz(const Block* __restrict__ blocks,
     const uint32_t lastend = nstart ? readlane(gend, nstart - 1) : grem;
     const uint32_t grem2 = lastend - c;
     const uint32_t g2 = g + nstart;
-    {  // the next chunk's bytes, requested now
-      const uint32_t tn = t + c + lane;
-      tv_n = tn < nt ? tokb[tn] : 0u;
-      sv_n = tn < nt ? symb[tn] : 0u;
-      gw_n = g2 + lane < ng ? grpb[g2 + lane] : 0u;
+    {  // the next chunk's bytes, requested now (at clamped indexes: no branch around the loads; nt >= 1 here)
+      const uint32_t tn = t + c + lane, gn = g2 + lane;
+      const uint32_t tv = tokb[min(tn, nt - 1u)], sv = symb[min(tn, nt - 1u)];
+      const uint32_t gv = grpb[ng ? min(gn, ng - 1u) : 0u];
+      tv_n = tn < nt ? tv : 0u;
+      sv_n = tn < nt ? sv : 0u;
+      gw_n = gn < ng ? gv : 0u;
     }
     // lengths: L >= 15 has L-15 in 3 ext bytes, then M-4 >= 15 has M-19 in 2 (src/codec2.h:178-180); offsets: b
     // raw bits for symbol b (0..19), or one of the group before's last three (250..252). One scan carries the
