@@ -174,12 +174,15 @@ __global__ __launch_bounds__(64) void k_dec_streams(const Block* __restrict__ bl
   const uint32_t bits4 = (body[lane >> 1] >> ((lane & 1u) * 4u)) & 15u;
   const uint32_t rank0 = wave_scan_excl((uint32_t)__popc(bits4));
   uint32_t f[4], fsum = 0;
-  {
-    uint32_t r = rank0;
+  {  // the lane's (at most 4) u16 frequencies in one 8-byte load (the lane lengths follow the table: in bounds)
+    uint64_t fw;
+    __builtin_memcpy(&fw, body + 32 + 2 * rank0, 8);
+    uint32_t r = 0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      f[k] = 0;
-      if ((bits4 >> k) & 1u) { f[k] = (uint32_t)body[32 + 2 * r] | (uint32_t)body[33 + 2 * r] << 8; ++r; }
+      const bool on = (bits4 >> k) & 1u;
+      f[k] = on ? (uint32_t)(fw >> (16u * r)) & 0xFFFFu : 0u;
+      r += on ? 1u : 0u;
       fsum += f[k];
     }
   }
@@ -191,7 +194,8 @@ __global__ __launch_bounds__(64) void k_dec_streams(const Block* __restrict__ bl
   // lane lengths: each at least 4 bytes, together inside the body (src/codec.h:185-187)
   const uint32_t lbase = 32 + 2 * np;
   const uint32_t data0 = lbase + 4 * 64;
-  const uint32_t ll = ld32u(body + lbase + 4 * lane);
+  uint32_t ll;
+  __builtin_memcpy(&ll, body + lbase + 4 * lane, 4);
   // every lane at least 4 bytes and all inside the body (src/codec.h:185-187); with each length capped at the
   // body size (< 2^25, a payload is smaller than its block) the 32-bit running sum cannot wrap
   if (ballot(ll < 4 || ll > len - data0 || len > (1u << 25))) { if (lane == 0) atomicOr(&err[b], E_LANE); return; }
@@ -224,7 +228,8 @@ __global__ __launch_bounds__(64) void k_dec_streams(const Block* __restrict__ bl
   const uint8_t* lp = body + data0 + lend - ll;  // this lane's bytes
   const uint32_t lnl = ll;
   const uint8_t* lim = p + blk.comp_len;
-  uint32_t x = ld32u(lp);
+  uint32_t x;
+  __builtin_memcpy(&x, lp, 4);  // (ll >= 4: inside the lane)
   uint32_t pos = 4;  // bytes of the lane consumed; the window c0..c3 holds bytes [pos, pos + 16)
   uint32_t c0, c1, c2, c3, n0, n1, n2, n3, cd, nd;  // (n0..n3: the 16 bytes after the window, shifted by nd)
   load16(lp, pos, lim, c0, c1, c2, c3, cd);

@@ -119,8 +119,10 @@ __device__ inline void wave_fence() {
 }
 
 // ---- byte-level helpers
-__device__ inline uint32_t ld32u(const uint8_t* p) {  // 4 bytes at any address, little endian
-  return (uint32_t)p[0] | (uint32_t)p[1] << 8 | (uint32_t)p[2] << 16 | (uint32_t)p[3] << 24;
+__device__ inline uint32_t ld32u(const uint8_t* p) {  // 4 bytes at any address, little endian (one unaligned load)
+  uint32_t v;
+  __builtin_memcpy(&v, p, 4);
+  return v;
 }
 
 }  // namespace zk
