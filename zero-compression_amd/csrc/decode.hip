@@ -11,14 +11,6 @@
 
 namespace zk {
 
-// exactly n (0..8) bytes at s, little endian, nothing past s + n read
-__device__ inline uint64_t load8n(const uint8_t* s, uint32_t n) {
-  if (n >= 8) { uint64_t v; __builtin_memcpy(&v, s, 8); return v; }
-  uint64_t v = 0;
-  for (uint32_t k = 0; k < n; ++k) v |= (uint64_t)s[k] << (8 * k);
-  return v;
-}
-
 // a wavefront copies n bytes together (16 bytes per lane per step)
 __device__ inline void wave_copy(uint8_t* d, const uint8_t* s, uint32_t n) {
   typedef uint32_t v4 __attribute__((ext_vector_type(4)));

@@ -25,12 +25,8 @@ constexpr uint32_t PARSE_WALKERS = 512;  // parse segments per block (at most; e
 __device__ inline uint32_t ld32(const uint8_t* p) { uint32_t v; __builtin_memcpy(&v, p, 4); return v; }
 __device__ inline uint64_t ld64(const uint8_t* p) { uint64_t v; __builtin_memcpy(&v, p, 8); return v; }
 
-// hash of the 5 bytes at p (kernels/zero.cl:49-58); p + 4 <= n, the byte past the block reads as 0
-__device__ inline uint32_t hash5(const uint8_t* inb, uint32_t n, uint32_t p) {
-  const uint32_t v = ld32(inb + p);
-  const uint32_t w = p + 4 < n ? (uint32_t)inb[p + 4] : 0u;
-  return (v * 2654435761u) ^ (w * 0x9E3779B1u + (w >> 3));
-}
+// hash of 5 bytes: v the first 4 (little endian), w the fifth, read as 0 past the block (kernels/zero.cl:49-58)
+__device__ inline uint32_t hash5v(uint32_t v, uint32_t w) { return (v * 2654435761u) ^ (w * 0x9E3779B1u + (w >> 3)); }
 
 // common prefix of the bytes at a and b (a < b), up to maxlen; b + maxlen <= n keeps every read in the block
 __device__ inline uint32_t prefix_len(const uint8_t* inb, uint32_t a, uint32_t b, uint32_t maxlen) {
@@ -78,8 +74,7 @@ __device__ inline uint64_t ld8z_small(const uint8_t* inb, uint32_t n, uint32_t q
 }
 // hash5 of position q + j from the 8 bytes x at q (j < 4)
 __device__ inline uint32_t hash5_of(uint64_t x, uint32_t j) {
-  const uint32_t v = (uint32_t)(x >> (8 * j)), w = (uint32_t)(x >> (8 * j + 32)) & 0xFFu;
-  return (v * 2654435761u) ^ (w * 0x9E3779B1u + (w >> 3));
+  return hash5v((uint32_t)(x >> (8 * j)), (uint32_t)(x >> (8 * j + 32)) & 0xFFu);
 }
 
 // ---------------------------------------------------------------- 1. hash, count, scatter
@@ -438,8 +433,7 @@ __device__ inline void load_window(const uint8_t* inb, uint32_t n, uint32_t p, u
 // the key of the position whose window is w (hash5 of its first 5 bytes, the fifth read as 0 past the block)
 template <int NW>
 __device__ inline uint32_t window_key(const uint32_t (&w)[NW], uint32_t keybits) {
-  const uint32_t x = w[1] & 0xFFu;
-  return ((w[0] * 2654435761u) ^ (x * 0x9E3779B1u + (x >> 3))) >> (32 - keybits);
+  return hash5v(w[0], w[1] & 0xFFu) >> (32 - keybits);
 }
 
 // the number of leading bytes where a and b agree (4 * NW when all do)

@@ -108,7 +108,6 @@ __device__ inline uint32_t scan_max(uint32_t v) {  // inclusive (values >= 0: a 
   v = max(v, dpp0<0x143, 0xC>(v));
   return v;
 }
-__device__ inline uint32_t sum_all(uint32_t v) { return __builtin_amdgcn_readlane(scan_add(v), 63); }
 __device__ inline uint32_t max_all(uint32_t v) { return __builtin_amdgcn_readlane(scan_max(v), 63); }
 
 // ordering of global-memory writes and later reads among the lanes of one wavefront (a compiler barrier; the
