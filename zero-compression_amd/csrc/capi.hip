@@ -123,10 +123,11 @@ struct EncLayout {
 struct EncShape {
   uint32_t blog, hbits, keybits, walkers;
 };
-// the parse's segments per block: at most PARSE_WALKERS, each at least 32 positions, and the parse's LDS (the
-// chain bitmap, 3 words and a 16-word window per walker) within 160 KiB
+// the parse's segments per block: one per PARSE_SPAN positions but at least PARSE_WALKERS_MIN, each at least 32
+// positions, and the parse's LDS (the chain bitmap, 3 words and a 16-word window per walker) within 160 KiB.
+// Fewer, longer segments cost more serial walking; more, shorter ones cost more joins (a long match crosses many).
 uint32_t walkers_of(uint32_t blog) {
-  uint32_t w = std::min<uint32_t>(PARSE_WALKERS, (1u << blog) / 32);
+  uint32_t w = std::min<uint32_t>(std::max<uint32_t>(PARSE_WALKERS_MIN, (1u << blog) / PARSE_SPAN), (1u << blog) / 32);
   while (w > 1 && ((size_t)(1u << blog) / 32 + 19 * (size_t)w) * 4 > 160 * 1024) w >>= 1;
   return w;
 }

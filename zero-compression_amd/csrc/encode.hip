@@ -20,7 +20,8 @@ namespace zk {
 
 #define KEY_TAG_BITS 12u
 __host__ __device__ inline uint32_t hbits_of(uint32_t blog) { return blog - 3u < 15u ? blog - 3u : 15u; }
-constexpr uint32_t PARSE_WALKERS = 512;  // parse segments per block (at most; each at least 32 positions)
+constexpr uint32_t PARSE_SPAN = 1024;       // default parse segment: positions per walker
+constexpr uint32_t PARSE_WALKERS_MIN = 128;  // default walkers per block, at least (each segment still >= 32 positions)
 
 __device__ inline uint32_t ld32(const uint8_t* p) { uint32_t v; __builtin_memcpy(&v, p, 4); return v; }
 __device__ inline uint64_t ld64(const uint8_t* p) { uint64_t v; __builtin_memcpy(&v, p, 8); return v; }
