@@ -81,46 +81,55 @@ __device__ inline void load16(const uint8_t* lp, uint32_t at, const uint8_t* lim
   __builtin_memcpy(v, a, 16);
   v0 = v[0]; v1 = v[1]; v2 = v[2]; v3 = v[3];
 }
-__device__ inline void shift16(uint32_t& v0, uint32_t& v1, uint32_t& v2, uint32_t& v3, uint32_t d) {
+// a when the mask m is all ones, b when it is zero (a bitwise select: no branch)
+__device__ inline uint32_t msel(uint32_t m, uint32_t a, uint32_t b) { return (a & m) | (b & ~m); }
+// bytes [d, d + 16) of the 32 bytes w0..w7 (d <= 16; bytes past w7 read as 0), selected by the bits of d / 4
+__device__ inline void realign16(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3, uint32_t w4, uint32_t w5,
+                                 uint32_t w6, uint32_t w7, uint32_t d, uint32_t& v0, uint32_t& v1, uint32_t& v2,
+                                 uint32_t& v3) {
   const uint32_t q = d >> 2, r = d & 3u;
-  const uint32_t e0 = q == 0 ? v0 : q == 1 ? v1 : q == 2 ? v2 : q == 3 ? v3 : 0u;
-  const uint32_t e1 = q == 0 ? v1 : q == 1 ? v2 : q == 2 ? v3 : 0u;
-  const uint32_t e2 = q == 0 ? v2 : q == 1 ? v3 : 0u;
-  const uint32_t e3 = q == 0 ? v3 : 0u;
+  const uint32_t m1 = 0u - (q & 1u), m2 = 0u - ((q >> 1) & 1u), m4 = 0u - (q >> 2);
+  const uint32_t t0 = msel(m1, w1, w0), t1 = msel(m1, w2, w1), t2 = msel(m1, w3, w2), t3 = msel(m1, w4, w3),
+                 t4 = msel(m1, w5, w4), t5 = msel(m1, w6, w5), t6 = msel(m1, w7, w6);
+  const uint32_t u0 = msel(m2, t2, t0), u1 = msel(m2, t3, t1), u2 = msel(m2, t4, t2), u3 = msel(m2, t5, t3),
+                 u4 = msel(m2, t6, t4);
+  const uint32_t e0 = msel(m4, w4, u0), e1 = msel(m4, w5, u1), e2 = msel(m4, w6, u2), e3 = msel(m4, w7, u3),
+                 e4 = msel(m4, 0u, u4);
   v0 = __builtin_amdgcn_alignbyte(e1, e0, r);
   v1 = __builtin_amdgcn_alignbyte(e2, e1, r);
   v2 = __builtin_amdgcn_alignbyte(e3, e2, r);
-  v3 = __builtin_amdgcn_alignbyte(0u, e3, r);
+  v3 = __builtin_amdgcn_alignbyte(e4, e3, r);
+}
+__device__ inline void shift16(uint32_t& v0, uint32_t& v1, uint32_t& v2, uint32_t& v3, uint32_t d) {
+  realign16(v0, v1, v2, v3, 0u, 0u, 0u, 0u, d, v0, v1, v2, v3);
 }
 
-// one rANS decoding step of a lane: the symbol of state x (slot table, then its freq | cum << 16) goes to *out; the
-// renormalisation takes 0, 1 or 2 bytes from the front of the lane's next bytes blo:bhi (first byte lowest), which
-// then move on by as many, and o counts them
-__device__ inline void rans_step(const uint8_t* slot, const uint32_t* fc, uint8_t* out, bool valid, uint32_t& blo,
-                                 uint32_t& bhi, uint32_t& x, uint32_t& o) {
+// one rANS decoding step of a lane: the symbol of state x (slot table, then its freq | cum << 16) goes to *out. The
+// lane's next bytes are w, big-endian (the first byte in the top 8 bits); the renormalisation shifts 0, 1 or 2 of
+// them into the state: two below 2^15, one below 2^23, read off the leading zeros of the new state (those of
+// xs | 2^11, at most 20, so that the count is 2 for every state below 2^15). ob counts the bits taken.
+__device__ inline void rans_step(const uint8_t* slot, const uint32_t* fc, uint8_t* out, bool valid, uint64_t& w,
+                                 uint32_t& x, uint32_t& ob) {
   const uint32_t sl = x & 4095u;
   const uint32_t sy = slot[sl];
   const uint32_t e = fc[sy];
   *out = (uint8_t)sy;
   const uint32_t xs = (e & 0xFFFFu) * (x >> 12) + sl - (e >> 16);
-  const uint32_t need = valid ? (xs < (1u << 15) ? 1u : 0u) + (xs < (1u << 23) ? 1u : 0u) : 0u;
-  const uint32_t s8 = 8u * need;
-  const uint32_t t16 = ((blo & 0xFFu) << 8) | ((blo >> 8) & 0xFFu);  // first byte high
-  const uint32_t xn = (xs << s8) | (t16 >> (16u - s8));
-  x = valid ? xn : x;
-  blo = __builtin_amdgcn_alignbyte(bhi, blo, need);
-  bhi >>= s8;
-  o += need;
+  const uint32_t lz = (uint32_t)__builtin_clz(xs | 0x800u);  // 0..20
+  const uint32_t s8 = valid ? __builtin_elementwise_sub_sat(lz, 1u) & 24u : 0u;
+  const uint64_t xw = ((uint64_t)xs << 32 | (uint32_t)(w >> 32)) << s8;
+  x = (uint32_t)(xw >> 32);
+  w <<= s8;
+  ob += s8;
 }
-// bytes [o, o + 8) of the 16-byte window c0..c3 (o <= 8) as blo:bhi
-__device__ inline void window8(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t o, uint32_t& blo,
-                               uint32_t& bhi) {
+// bytes [o, o + 8) of the 16-byte window c0..c3 (o <= 8), big-endian (the first byte in the top 8 bits)
+__device__ inline uint64_t window8(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t o) {
   const uint32_t q = o >> 2, r = o & 3u;
   const uint32_t w0 = q == 0 ? c0 : q == 1 ? c1 : c2;
   const uint32_t w1 = q == 0 ? c1 : q == 1 ? c2 : c3;
   const uint32_t w2 = q == 0 ? c2 : q == 1 ? c3 : 0u;  // (q = 2 comes with r = 0)
-  blo = __builtin_amdgcn_alignbyte(w1, w0, r);
-  bhi = __builtin_amdgcn_alignbyte(w2, w1, r);
+  const uint32_t lo = __builtin_amdgcn_alignbyte(w1, w0, r), hi = __builtin_amdgcn_alignbyte(w2, w1, r);
+  return (uint64_t)__builtin_bswap32(lo) << 32 | __builtin_bswap32(hi);
 }
 
 // ---------------------------------------------------------------- the streams: one wavefront per (block, stream)
@@ -231,24 +240,18 @@ __global__ __launch_bounds__(64) void k_dec_streams(const Block* __restrict__ bl
   uint32_t b0 = 0;
   for (; b0 + 8 <= nfull; b0 += 8) {
     // 4 steps from the window's first 8 bytes (they take at most 8), 4 from the 8 after what those took
-    uint32_t o = 0, blo = c0, bhi = c1;
+    uint32_t ob = 0;
+    uint64_t w = window8(c0, c1, c2, c3, 0u);
 #pragma unroll
-    for (uint32_t k = 0; k < 4; ++k) rans_step(slot, fc, stg + k * 64 + lane, true, blo, bhi, x, o);
-    window8(c0, c1, c2, c3, o, blo, bhi);
+    for (uint32_t k = 0; k < 4; ++k) rans_step(slot, fc, stg + k * 64 + lane, true, w, x, ob);
+    w = window8(c0, c1, c2, c3, ob >> 3);
 #pragma unroll
-    for (uint32_t k = 4; k < 8; ++k) rans_step(slot, fc, stg + k * 64 + lane, true, blo, bhi, x, o);
-    // the window moves on by o bytes: bytes [o, o + 16) of c0..c3 then n0..n3 (which arrived meanwhile)
-    shift16(n0, n1, n2, n3, nd);
-    const uint32_t q = o >> 2, r = o & 3u;
-    const uint32_t e0 = q == 0 ? c0 : q == 1 ? c1 : q == 2 ? c2 : q == 3 ? c3 : n0;
-    const uint32_t e1 = q == 0 ? c1 : q == 1 ? c2 : q == 2 ? c3 : q == 3 ? n0 : n1;
-    const uint32_t e2 = q == 0 ? c2 : q == 1 ? c3 : q == 2 ? n0 : q == 3 ? n1 : n2;
-    const uint32_t e3 = q == 0 ? c3 : q == 1 ? n0 : q == 2 ? n1 : q == 3 ? n2 : n3;
-    const uint32_t e4 = q == 0 ? n0 : q == 1 ? n1 : q == 2 ? n2 : q == 3 ? n3 : 0u;
-    c0 = __builtin_amdgcn_alignbyte(e1, e0, r);
-    c1 = __builtin_amdgcn_alignbyte(e2, e1, r);
-    c2 = __builtin_amdgcn_alignbyte(e3, e2, r);
-    c3 = __builtin_amdgcn_alignbyte(e4, e3, r);
+    for (uint32_t k = 4; k < 8; ++k) rans_step(slot, fc, stg + k * 64 + lane, true, w, x, ob);
+    const uint32_t o = ob >> 3;
+    // the window moves on by o bytes: bytes [o, o + 16) of c0..c3 then n0..n3 (which arrived meanwhile; their load
+    // was moved back by nd bytes only near the payload's end)
+    if (ballot(nd != 0u)) shift16(n0, n1, n2, n3, nd);
+    realign16(c0, c1, c2, c3, n0, n1, n2, n3, o, c0, c1, c2, c3);
     pos += o;
     load16(lp, pos + 16, lim, n0, n1, n2, n3, nd);
     // these 512 symbols to their place, after the load: a wait for that load then does not wait for these stores
@@ -257,15 +260,14 @@ __global__ __launch_bounds__(64) void k_dec_streams(const Block* __restrict__ bl
     wave_fence();
   }
   if (b0 < nsteps) {  // the last (at most 8) steps, within the window
-    uint32_t o = 0, blo = c0, bhi = c1;
+    uint32_t ob = 0;
+    uint64_t w = window8(c0, c1, c2, c3, 0u);
 #pragma unroll
-    for (uint32_t k = 0; k < 4; ++k)
-      rans_step(slot, fc, stg + k * 64 + lane, (b0 + k) * 64 + lane < n, blo, bhi, x, o);
-    window8(c0, c1, c2, c3, o, blo, bhi);
+    for (uint32_t k = 0; k < 4; ++k) rans_step(slot, fc, stg + k * 64 + lane, (b0 + k) * 64 + lane < n, w, x, ob);
+    w = window8(c0, c1, c2, c3, ob >> 3);
 #pragma unroll
-    for (uint32_t k = 4; k < 8; ++k)
-      rans_step(slot, fc, stg + k * 64 + lane, (b0 + k) * 64 + lane < n, blo, bhi, x, o);
-    pos += o;
+    for (uint32_t k = 4; k < 8; ++k) rans_step(slot, fc, stg + k * 64 + lane, (b0 + k) * 64 + lane < n, w, x, ob);
+    pos += ob >> 3;
     wave_fence();
     wave_copy_lds(dest + b0 * 64, stg, n - b0 * 64);
     wave_fence();
