@@ -95,6 +95,55 @@ def test_crc32_ranges(gpu):
         gpu.crc32_ranges(0, [(0, 1)])  # (the decompress replaced the compress input)
 
 
+def lz_from_tokens(rng, n, short_groups):
+    """bytes and their (match position, length, offset) tokens, built by running random tokens: offsets are mostly
+    taken again from the last three tokens (repeats, chains of them), and either short (groups close after a few
+    tokens) or long (groups run to 255 tokens, across many chunks of 64 tokens)"""
+    out, toks, last, recent = bytearray(), [], [1, 4, 8], [1, 4, 8]
+    gstart, gt = 0, 0  # (the open group, tracked only to steer the offsets towards repeats)
+    while True:
+        L = rng.choice([0, 0, 1, 3, 7, 15, 16, 40]) if rng.random() < 0.9 else rng.randrange(300)
+        L = max(L, 1 - len(out))
+        M = 4 + (rng.randrange(12) if rng.random() < 0.8 else rng.randrange(300))
+        i = len(out) + L
+        if i + M > n:
+            break
+        out += bytes(rng.randrange(256) for _ in range(L))
+        if rng.random() < 0.6:
+            d = rng.choice(last + recent)
+        elif short_groups:
+            d = rng.randrange(1, 24)
+        else:
+            d = rng.randrange(min(i, 2000), i + 1)
+        d = max(1, min(d, i))
+        if gt == 255 or (gt and i - d < len(out) - L and i - d + min(M, d) > gstart):
+            gstart, gt, recent = len(out) - L, 0, last
+        for k in range(M):
+            out.append(out[i - d + k])
+        toks.append((i, M, d))
+        last = [d] + last[:2]
+        gt += 1
+    out += bytes(rng.randrange(256) for _ in range(n - len(out)))
+    return bytes(out), toks
+
+
+@pytest.mark.parametrize("bl", [16, 18])
+def test_decode_repeat_chains(gpu, bl):
+    """blocks full of repeated offsets (chains of repeats inside a chunk of tokens, repeats of a group that began
+    chunks earlier) decode like the sequential decoder"""
+    rng = random.Random(bl)
+    n = 1 << bl
+    blocks, want = [], b""
+    for short in (True, False, True, False):
+        d, toks = lz_from_tokens(rng, n, short)
+        payload = O.pack_tokens(d, toks)
+        rc, back = O.decompress_block(payload, n)
+        assert rc == 0 and back == d
+        blocks.append((1, n, payload))
+        want += d
+    assert gpu.decompress(blocks, bl) == want
+
+
 def test_damaged_payloads(gpu):
     """damage is refused or decodes like the sequential decoder; it never faults"""
     d = corpus(small=True)["mixed"][: 1 << 16]

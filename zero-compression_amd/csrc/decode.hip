@@ -578,6 +578,7 @@ __global__ __launch_bounds__(64) void k_dec_lz(const Block* __restrict__ blocks,
                                                uint8_t* out, uint32_t* __restrict__ err) {
   __shared__ uint32_t sflag[64];  // token k of the chunk starts a group
   __shared__ uint32_t sgs[64];    // output start of the chunk's k-th group
+  __shared__ uint32_t sgk[64];    // first token (in the chunk) of the chunk's k-th group
   __shared__ uint32_t sop[64], sL[64], sM[64], sD[64], sli[64], scs[64], srg[64], sown[64];
   __shared__ uint8_t ring[RING + 16];  // the block's most recent output
   const Geo G = geo_of(blog);
@@ -684,17 +685,25 @@ __global__ __launch_bounds__(64) void k_dec_lz(const Block* __restrict__ blocks,
         D = (1u << nb_) + ((w >> (bq & 7u)) & ((1u << nb_) - 1u));
       }
     }
-    // repeated offsets, group by group: recent = the last three offsets as of the end of the group before
-    for (uint32_t q = 0; q < ngc; ++q) {
-      if (q > 0 || grem == 0) { r0 = l0; r1 = l1; r2 = l2; }  // group q starts here
-      const bool inq = on && rg == q;
-      if (inq && rep && !bad) D = sv == 250u ? r0 : sv == 251u ? r1 : r2;
-      const uint64_t qm = ballot(inq);
-      if (!qm) continue;  // (every group of the chunk holds a token of it)
-      const uint32_t qb = 64u - (uint32_t)__builtin_clzll(qm), qa = (uint32_t)__builtin_ctzll(qm);
-      for (uint32_t x = qb - qa > 3u ? qb - 3u : qa; x < qb; ++x) {
-        const uint32_t v = readlane(D, x);
-        l2 = l1; l1 = l0; l0 = v;
+    // repeated offsets: the recent offsets of a group are those of the three tokens before its first one. So a
+    // repeat j (symbol 250 + j) in a group that starts in this chunk at token gsk takes the offset of token
+    // gsk - 1 - j: one of this chunk (resolved first when it is a repeat itself), or one of the last three before
+    // the chunk (l0..l2); in the group still open from the chunk before it takes r0..r2, that group's recent offsets
+    if (first) sgk[rg] = lane;
+    wave_fence();
+    const bool cont = rg == 0 && grem > 0;  // (a token of the group open from the chunk before)
+    const uint32_t gsk = cont ? 0u : sgk[rg & 63u];
+    if (ballot(on && rep && !bad)) {
+      const bool rp = on && rep && !bad;
+      const uint32_t j = rp ? sv - 250u : 0u;
+      const uint32_t ix = j - gsk;  // (for a source before the chunk: 0..2)
+      bool pend = rp && !cont && gsk > j;
+      if (rp) D = cont ? (j == 0 ? r0 : j == 1 ? r1 : r2) : (ix == 0 ? l0 : ix == 1 ? l1 : l2);
+      const uint32_t from = pend ? gsk - 1u - j : lane;
+      while (ballot(pend)) {  // one link of a chain of repeats per pass (each source is an earlier token)
+        const uint32_t v = (uint32_t)__shfl((int)D, (int)from, 64);
+        const uint32_t sp = (uint32_t)__shfl(pend ? 1 : 0, (int)from, 64);
+        if (pend && !sp) { D = v; pend = false; }
       }
     }
     const uint32_t tot = L + M;
@@ -715,6 +724,19 @@ __global__ __launch_bounds__(64) void k_dec_lz(const Block* __restrict__ blocks,
     sop[lane] = opk; sL[lane] = L; sM[lane] = M; sD[lane] = D; sli[lane] = lik; scs[lane] = ccs; srg[lane] = rg;
     wave_fence();
     if (grem2 && nstart) gs_open = readlane(opk, readlane(gbeg, nstart - 1));  // (the open group's first token)
+    {  // recent offsets carried on: those of the group that stays open (when it started here), then l0..l2
+      const uint32_t gl = readlane(gsk, c - 1);
+      if (!readlane(cont ? 1u : 0u, c - 1)) {
+        const uint32_t a0 = gl >= 1u ? readlane(D, gl - 1u) : l0;
+        const uint32_t a1 = gl >= 2u ? readlane(D, gl - 2u) : gl == 1u ? l0 : l1;
+        const uint32_t a2 = gl >= 3u ? readlane(D, gl - 3u) : gl == 2u ? l0 : gl == 1u ? l1 : l2;
+        r0 = a0; r1 = a1; r2 = a2;
+      }
+      const uint32_t b0 = readlane(D, c - 1u);
+      const uint32_t b1 = c >= 2u ? readlane(D, c - 2u) : l0;
+      const uint32_t b2 = c >= 3u ? readlane(D, c - 3u) : c == 2u ? l0 : l1;
+      l0 = b0; l1 = b1; l2 = b2;
+    }
     const uint32_t op_end = op + readlane(itot, 63);
     op = op_end;
     li += readlane(il, 63);
