@@ -55,32 +55,6 @@ __device__ inline uint64_t ballot(bool p) { return __ballot(p); }
 __device__ inline uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
 __device__ inline uint32_t popc64(uint64_t m) { return (uint32_t)__popcll(m); }
 
-// inclusive prefix sum over the 64 lanes
-__device__ inline uint32_t wave_scan_incl(uint32_t v) {
-  const uint32_t lane = lane_id();
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t t = __shfl_up(v, o, 64);
-    if (lane >= (uint32_t)o) v += t;
-  }
-  return v;
-}
-__device__ inline uint32_t wave_scan_excl(uint32_t v) { return wave_scan_incl(v) - v; }
-__device__ inline uint32_t wave_sum(uint32_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-__device__ inline uint32_t wave_max(uint32_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor(v, o, 64));
-  return v;
-}
-__device__ inline uint32_t wave_min(uint32_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor(v, o, 64));
-  return v;
-}
 // v in `lane` (the lane index must be the same in every lane: v_readlane)
 __device__ inline uint32_t readlane(uint32_t v, uint32_t lane) { return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane); }
 
@@ -109,6 +83,13 @@ __device__ inline uint32_t scan_max(uint32_t v) {  // inclusive (values >= 0: a 
   return v;
 }
 __device__ inline uint32_t max_all(uint32_t v) { return __builtin_amdgcn_readlane(scan_max(v), 63); }
+
+// whole-wavefront scans and reductions, by DPP (every lane of the wavefront must be active)
+__device__ inline uint32_t wave_scan_incl(uint32_t v) { return scan_add(v); }
+__device__ inline uint32_t wave_scan_excl(uint32_t v) { return scan_add(v) - v; }
+__device__ inline uint32_t wave_sum(uint32_t v) { return __builtin_amdgcn_readlane(scan_add(v), 63); }
+__device__ inline uint32_t wave_max(uint32_t v) { return max_all(v); }
+__device__ inline uint32_t wave_min(uint32_t v) { return ~max_all(~v); }
 
 // ordering of global-memory writes and later reads among the lanes of one wavefront (a compiler barrier; the
 // hardware keeps one wavefront's vector memory operations in order)
