@@ -762,8 +762,8 @@ __global__ __launch_bounds__(64) void k_enc_tokens(const Block* __restrict__ blo
     const uint32_t base = p0 + 8 * lane;
     const uint32_t bits = (cw >> (base & 31u)) & 0xFFu;  // which of this lane's 8 positions are on the chain
     // the lazy values: the match value after the lane's last comes from the next lane, or the next step's lane 0
-    // (the shuffle runs with every lane active: a lane that sits out gives the lane before it nothing)
-    const uint32_t from_next = (uint32_t)__shfl_down((int)cm[0], 1, 64);
+    // (the lane move runs with every lane active: a lane that sits out gives the lane before it nothing)
+    const uint32_t from_next = from_next_lane(cm[0]);
     const uint32_t next_step = p0 + 512 < n ? readlane(nm[0], 0) : 0u;
     const uint32_t after = lane == 63u ? next_step : from_next;
     uint32_t tmask = 0;
@@ -810,7 +810,7 @@ __global__ __launch_bounds__(64) void k_enc_tokens(const Block* __restrict__ blo
     const uint32_t qn = min(t + 64 + lane, nt - 1);
     const uint32_t lbn = LB[qn], Mn = TM[qn], Dn = TD[qn];
     const uint32_t lb = on ? lbr : 0u;
-    const uint32_t up = (uint32_t)__shfl_up((int)lbr, 1, 64);
+    const uint32_t up = from_prev_lane(lbr);
     const uint32_t lprev = on ? (lane ? up : lbprev) : 0u;
     const uint32_t L = lb - lprev;
     const uint32_t M = on ? Mr : 0u;

@@ -64,6 +64,8 @@ template <int CTRL, int ROW_MASK = 0xF>
 __device__ inline uint32_t dpp0(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xF, true);
 }
+__device__ inline uint32_t from_next_lane(uint32_t v) { return dpp0<0x130>(v); }  // wave_shl:1 (lane 63 reads 0)
+__device__ inline uint32_t from_prev_lane(uint32_t v) { return dpp0<0x138>(v); }  // wave_shr:1 (lane 0 reads 0)
 __device__ inline uint32_t scan_add(uint32_t v) {  // inclusive
   v += dpp0<0x111>(v);       // row_shr:1
   v += dpp0<0x112>(v);       // row_shr:2
