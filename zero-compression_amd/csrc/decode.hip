@@ -515,6 +515,11 @@ __device__ inline void ring_copy(uint8_t* ring, const uint8_t* o, const uint8_t*
 #pragma unroll
     for (uint32_t r = 0; r < RROUNDS; ++r) {
       const uint32_t c0 = base + 64 * r;
+      if (c0 >= C) {  // (uniform: no chunks left for this round)
+        d[r] = CDesc{0u, 0u, 0u, K_NONE << 8, 0u};
+        pre[r] = U128{0, 0};
+        continue;
+      }
       sown[lane] = 0;
       wave_fence();
       const uint32_t jk = ccs_lane <= c0 ? 0u : ccs_lane - c0;
