@@ -259,7 +259,20 @@ __device__ inline void sort16(uint32_t* s, uint32_t k) {
   for (int i = 0; i < 16; ++i)
     if ((uint32_t)i < k) s[i] = v[i];
 }
-// the k (17..64) values at s sorted by the whole wavefront, one per lane: a bitonic network over lane shuffles
+// the value of lane ^ j (j = 1, 2, ..., 32) without LDS traffic: DPP inside 16-lane rows (quad permutations, row
+// rotations: row_ror:n gives lane i the value of lane i - n of its row), the gfx950 permlane swaps across rows
+// (permlane16_swap(v, v) gives [v of lane i & ~16, v of lane i | 16], permlane32_swap likewise for bit 5)
+__device__ inline uint32_t xor_lane(uint32_t v, uint32_t j, uint32_t lane) {
+  switch (j) {
+    case 1: return dpp0<0xB1>(v);  // quad_perm [1,0,3,2]
+    case 2: return dpp0<0x4E>(v);  // quad_perm [2,3,0,1]
+    case 4: { const uint32_t up = dpp0<0x12C>(v), dn = dpp0<0x124>(v); return (lane & 4u) ? dn : up; }
+    case 8: return dpp0<0x128>(v);  // row_ror:8
+    case 16: { const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false); return (lane & 16u) ? r[0] : r[1]; }
+    default: { const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false); return (lane & 32u) ? r[0] : r[1]; }
+  }
+}
+// the k (17..64) values at s sorted by the whole wavefront, one per lane: a bitonic network over lane exchanges
 __device__ inline void sort64_wave(uint32_t* s, uint32_t k) {
   const uint32_t lane = lane_id();
   uint32_t v = lane < k ? s[lane] : 0xFFFFFFFFu;
@@ -267,7 +280,7 @@ __device__ inline void sort64_wave(uint32_t* s, uint32_t k) {
   for (uint32_t kk = 2; kk <= 64; kk <<= 1)
 #pragma unroll
     for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
-      const uint32_t o = (uint32_t)__shfl_xor((int)v, (int)j, 64);
+      const uint32_t o = xor_lane(v, j, lane);
       const bool keep_min = ((lane & j) == 0) == ((lane & kk) == 0);
       v = keep_min ? min(v, o) : max(v, o);
     }
