@@ -478,7 +478,9 @@ __device__ inline void match_slots(const uint8_t* __restrict__ inb, uint32_t n, 
   if (!act) p = 0;
   uint32_t w[NW], xw[NW];
   load_window<NW>(inb, n, p, w);
-  const bool run = act && p > 0 && w[0] == (uint32_t)inb[p - (p > 0)] * 0x01010101u;
+  // (the byte before p is read only where the window starts with 4 equal bytes: those lanes alone take the load)
+  bool run = false;
+  if (act && p > 0 && w[0] == (w[0] & 0xFFu) * 0x01010101u) run = inb[p - 1] == (w[0] & 0xFFu);
   if (lane < MX) load_window<NW>(inb, n, xp < count ? xp : 0u, xw);
   const uint32_t key = window_key<NW>(w, keybits);
   lpos[MX + lane] = act ? p : NOPOS;
