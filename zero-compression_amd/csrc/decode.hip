@@ -236,6 +236,7 @@ __global__ __launch_bounds__(64) void k_dec_streams(const Block* __restrict__ bl
   load16(lp, pos, lim, c0, c1, c2, c3, cd);
   shift16(c0, c1, c2, c3, cd);
   load16(lp, pos + 16, lim, n0, n1, n2, n3, nd);
+  wait_vm_all();  // (x, c and n are here: the step loop's head then waits for nothing from before the loop)
   const uint32_t nsteps = (n + 63) >> 6, nfull = n >> 6;  // (every lane has a symbol in the first nfull steps)
   uint32_t b0 = 0;
   for (; b0 + 8 <= nfull; b0 += 8) {
@@ -256,7 +257,11 @@ __global__ __launch_bounds__(64) void k_dec_streams(const Block* __restrict__ bl
     load16(lp, pos + 16, lim, n0, n1, n2, n3, nd);
     // these 512 symbols to their place, after the load: a wait for that load then does not wait for these stores
     wave_fence();
-    wave_copy_lds(dest + b0 * 64, stg, 512u);
+    {  // (512 bytes, 8 per lane: every lane stores, so the loop has no branch around the store)
+      uint64_t v;
+      __builtin_memcpy(&v, stg + lane * 8, 8);
+      __builtin_memcpy(dest + b0 * 64 + lane * 8, &v, 8);
+    }
     wave_fence();
   }
   if (b0 < nsteps) {  // the last (at most 8) steps, within the window
@@ -332,6 +337,12 @@ __device__ inline void st_partial(uint8_t* d, U128 v, uint32_t len) {
   if (len & 1u) *d = (uint8_t)w;
 }
 
+// the fields of token k of a chunk of tokens, as the copy needs them (one 32-byte LDS record: two 16-byte reads)
+struct alignas(16) TokF {
+  uint32_t opk, L, M, D;  // output start, literal length, match length, offset
+  uint32_t lik, cs, grp, pad;  // literal start, first copy chunk, group (from 0 in the chunk of tokens)
+};
+
 // Chunks of a token: its literals in 16-byte chunks, then its match in chunks that each read one stretch of
 // consecutive source bytes. A match whose offset D is at least its length M reads [ms - D, me - D) (ms, me: the
 // match's output range); one with 16 <= D < M repeats the period [ms - D, ms), each repetition cut into
@@ -351,9 +362,7 @@ __device__ inline uint32_t chunk_count(uint32_t L, uint32_t M, uint32_t D) {
 // earlier group), and stores exactly its length. Sources that straddle the token's first output byte take a second
 // load; a 16-byte read of output past the block's end goes byte by byte.
 __device__ inline void group_copy(uint8_t* o, const uint8_t* litb, uint32_t raw, bool in_group, uint32_t ccs_lane,
-                                  uint32_t cb, uint32_t ce, const uint32_t* sop, const uint32_t* sL,
-                                  const uint32_t* sM, const uint32_t* sD, const uint32_t* sli, const uint32_t* scs,
-                                  uint32_t* sown) {
+                                  uint32_t cb, uint32_t ce, const TokF* stk, uint32_t* sown) {
   const uint32_t lane = lane_id();
   for (uint32_t c0 = cb; c0 < ce; c0 += 64) {
     sown[lane] = 0;
@@ -366,7 +375,8 @@ __device__ inline void group_copy(uint8_t* o, const uint8_t* litb, uint32_t raw,
     const uint32_t cj = c0 + lane;
     const bool live = cj < ce;
     const uint32_t kk = live ? k : 0u;
-    const uint32_t opk = sop[kk], L = sL[kk], M = sM[kk], D = sD[kk], lik = sli[kk], cs = scs[kk];
+    const TokF tk = stk[kk];
+    const uint32_t opk = tk.opk, L = tk.L, M = tk.M, D = tk.D, lik = tk.lik, cs = tk.cs;
     const uint32_t w = cj - cs;
     const uint32_t nlit = (L + 15u) >> 4;
     const uint32_t ms = opk + L, me = ms + M;
@@ -417,7 +427,7 @@ __device__ inline void group_copy(uint8_t* o, const uint8_t* litb, uint32_t raw,
 // reads and writes only, and at the end the chunk's output goes to global memory in 16-byte stores. So a chunk of
 // tokens costs one round trip to memory instead of one per group.
 constexpr uint32_t RING = 4096;
-constexpr uint32_t RROUNDS = 2;  // copy chunks laid out at once: RROUNDS x 64
+constexpr uint32_t RROUNDS = 3;  // copy chunks laid out at once: RROUNDS x 64
 
 __device__ inline U128 ring_read16(const uint8_t* ring, uint32_t y) {
   U128 v;
@@ -449,27 +459,40 @@ struct CDesc {
 };
 enum : uint32_t { K_PRE = 0, K_RING = 1, K_SLOW = 2, K_NONE = 3 };
 
-__device__ inline CDesc describe(uint32_t cj, bool live, uint32_t k, const uint32_t* sop, const uint32_t* sL,
-                                 const uint32_t* sM, const uint32_t* sD, const uint32_t* sli, const uint32_t* scs,
-                                 const uint32_t* srg, const uint8_t* o, const uint8_t* litb, uint32_t raw, uint32_t lo,
-                                 const uint8_t*& src) {
+__device__ inline CDesc describe(uint32_t cj, bool live, uint32_t k, const TokF* stk, const uint8_t* o,
+                                 const uint8_t* litb, uint32_t raw, uint32_t lo, const uint8_t*& src) {
   CDesc d;
   const uint32_t kk = live ? k : 0u;
-  const uint32_t opk = sop[kk], L = sL[kk], M = sM[kk], D = sD[kk], lik = sli[kk], cs = scs[kk], grp = srg[kk];
+  const TokF tk = stk[kk];
+  const uint32_t opk = tk.opk, L = tk.L, M = tk.M, D = tk.D, lik = tk.lik, cs = tk.cs, grp = tk.grp;
   const uint32_t w = cj - cs;
   const uint32_t nlit = (L + 15u) >> 4;
-  const uint32_t ms = opk + L, me = ms + M;
+  const uint32_t ms = opk + L;
   const bool litc = w < nlit;
-  const bool pat = !litc && D < 16u;
-  uint32_t rep = 0, within = litc ? w : w - nlit;
-  if (ballot(live && !litc && D >= 16u && D < M)) {
-    if (!litc && D >= 16u && D < M) divmod_small(within, (D + 15u) >> 4, rep, within);
+  const uint32_t wm = w - nlit;  // (a match chunk: its index in the match)
+  // literals, or a match that does not overlap its output (D >= M): 16-byte chunks of [ms - D, ms - D + M)
+  uint32_t y0 = litc ? opk + 16u * w : ms + 16u * wm;
+  uint32_t cend = litc ? ms : ms + M;
+  uint32_t s0 = litc ? y0 : y0 - D;
+  bool pat = false;
+  uint32_t ph = 0;
+  if (ballot(live && !litc && D < M)) {  // a match that overlaps its output: the period pattern, or repetitions
+    if (live && !litc && D < M) {
+      if (D < 16u) {
+        pat = true;
+        s0 = ms - D;
+        ph = mod_small(16u * wm, D);
+      } else {
+        uint32_t rep, within;
+        divmod_small(wm, (D + 15u) >> 4, rep, within);
+        const uint32_t base = ms + rep * D;
+        y0 = base + 16u * within;
+        cend = min(cend, base + D);
+        s0 = y0 - (rep + 1u) * D;
+      }
+    }
   }
-  const uint32_t base = litc ? opk : ms + rep * D;
-  const uint32_t y0 = base + 16u * within;
-  const uint32_t cend = litc ? ms : pat ? me : min(me, base + D);
   const uint32_t len = live ? min(16u, cend - y0) : 0u;
-  const uint32_t s0 = litc ? y0 : pat ? ms - D : y0 - (rep + 1u) * D;
   const uint32_t need = pat ? D : len;
   uint32_t kind;
   if (!live) kind = K_NONE;
@@ -479,7 +502,6 @@ __device__ inline CDesc describe(uint32_t cj, bool live, uint32_t k, const uint3
   else if (s0 + need > lo || s0 + 16u > raw) kind = K_SLOW;            // across the window's edge, or the block's end
   else kind = K_PRE;                                                   // older output, in global memory
   src = s0 >= opk ? litb + lik + (s0 - opk) : o + s0;
-  const uint32_t ph = pat ? mod_small(16u * within, D) : 0u;
   d.y0 = y0;
   d.len = len;
   d.s0 = s0;
@@ -490,8 +512,8 @@ __device__ inline CDesc describe(uint32_t cj, bool live, uint32_t k, const uint3
 
 // a copy chunk byte by byte: its source bytes from the literals, the window or global memory
 __device__ inline void copy_slow(const CDesc& d, uint8_t* ring, const uint8_t* o, const uint8_t* litb, uint32_t lo,
-                                 const uint32_t* sop, const uint32_t* sli) {
-  const uint32_t opk = sop[d.k], lik = sli[d.k];
+                                 const TokF* stk) {
+  const uint32_t opk = stk[d.k].opk, lik = stk[d.k].lik;
   const bool pat = (d.flags >> 10) & 1u;
   const uint32_t ph = (d.flags >> 11) & 15u, D = d.flags >> 16;
   for (uint32_t i = 0; i < d.len; ++i) {
@@ -505,9 +527,7 @@ __device__ inline void copy_slow(const CDesc& d, uint8_t* ring, const uint8_t* o
 
 // the chunk of tokens' copy chunks [0, C) through the window; lo: the oldest output position the window holds
 __device__ inline void ring_copy(uint8_t* ring, const uint8_t* o, const uint8_t* litb, uint32_t raw, uint32_t lo,
-                                 uint32_t C, bool on, uint32_t ccs_lane, const uint32_t* sop, const uint32_t* sL,
-                                 const uint32_t* sM, const uint32_t* sD, const uint32_t* sli, const uint32_t* scs,
-                                 const uint32_t* srg, uint32_t* sown) {
+                                 uint32_t C, bool on, uint32_t ccs_lane, const TokF* stk, uint32_t* sown) {
   const uint32_t lane = lane_id();
   for (uint32_t base = 0; base < C; base += 64 * RROUNDS) {
     CDesc d[RROUNDS];
@@ -529,7 +549,7 @@ __device__ inline void ring_copy(uint8_t* ring, const uint8_t* o, const uint8_t*
       wave_fence();
       const uint32_t cj = c0 + lane;
       const uint8_t* src;
-      d[r] = describe(cj, cj < C, k, sop, sL, sM, sD, sli, scs, srg, o, litb, raw, lo, src);
+      d[r] = describe(cj, cj < C, k, stk, o, litb, raw, lo, src);
       pre[r] = ld128(((d[r].flags >> 8) & 3u) == K_PRE ? src : litb);
     }
     // in chunk order, which is group order: round by round, and inside a round group by group (a chunk reads only
@@ -554,11 +574,14 @@ __device__ inline void ring_copy(uint8_t* ring, const uint8_t* o, const uint8_t*
         }
         ring_write(ring, d[r].y0, v, d[r].len, act && kind != K_SLOW);
         if (ballot(act && kind == K_SLOW)) {
-          if (act && kind == K_SLOW) copy_slow(d[r], ring, o, litb, lo, sop, sli);
+          if (act && kind == K_SLOW) copy_slow(d[r], ring, o, litb, lo, stk);
         }
         wave_fence();  // (this group's bytes are in the window before the next group reads them)
       }
     }
+    // (every load of this batch has been used; saying so keeps the compiler from waiting at the loop's head, which
+    // would also wait for the stores and loads issued before the loop)
+    wait_vm_all();
   }
 }
 
@@ -584,7 +607,8 @@ __global__ __launch_bounds__(64) void k_dec_lz(const Block* __restrict__ blocks,
   __shared__ uint32_t sflag[64];  // token k of the chunk starts a group
   __shared__ uint32_t sgs[64];    // output start of the chunk's k-th group
   __shared__ uint32_t sgk[64];    // first token (in the chunk) of the chunk's k-th group
-  __shared__ uint32_t sop[64], sL[64], sM[64], sD[64], sli[64], scs[64], srg[64], sown[64];
+  __shared__ TokF stk[64];          // the chunk of tokens' fields
+  __shared__ uint32_t sown[64];
   __shared__ uint8_t ring[RING + 16];  // the block's most recent output
   const Geo G = geo_of(blog);
   const uint32_t b = blockIdx.x;
@@ -615,6 +639,7 @@ __global__ __launch_bounds__(64) void k_dec_lz(const Block* __restrict__ blocks,
   uint32_t g = 0, grem = 0, gs_open = 0;  // groups started so far; tokens left in the open group; its output start
   uint32_t r0 = 1, r1 = 4, r2 = 8, l0 = 1, l1 = 4, l2 = 8;
   uint32_t ring_lo = 0;  // output before this is not in the window (it went to global memory another way)
+  uint32_t fl_a = 0, fl_e = 0;  // output [fl_a, fl_e) is in the window and not yet in global memory
   uint32_t fail = 0;
   uint32_t tv_n = lane < nt ? tokb[lane] : 0u, sv_n = lane < nt ? symb[lane] : 0u;
   uint32_t gw_n = lane < ng ? grpb[lane] : 0u;
@@ -640,13 +665,18 @@ __global__ __launch_bounds__(64) void k_dec_lz(const Block* __restrict__ blocks,
     const uint32_t lastend = nstart ? readlane(gend, nstart - 1) : grem;
     const uint32_t grem2 = lastend - c;
     const uint32_t g2 = g + nstart;
-    {  // the next chunk's bytes, requested now (at clamped indexes: no branch around the loads; nt >= 1 here)
-      const uint32_t tn = t + c + lane, gn = g2 + lane;
-      const uint32_t tv = tokb[min(tn, nt - 1u)], sv = symb[min(tn, nt - 1u)];
-      const uint32_t gv = grpb[ng ? min(gn, ng - 1u) : 0u];
-      tv_n = tn < nt ? tv : 0u;
-      sv_n = tn < nt ? sv : 0u;
-      gw_n = gn < ng ? gv : 0u;
+    // The previous chunk's output leaves the window here, at the start of this chunk's parse rather than at the end
+    // of its own copy, and the next chunk's token bytes are requested (clamped indexes, no branch around the loads;
+    // lanes past nt or ng are masked at the loop's head: lane >= c, g + lane >= ng). They complete together with this
+    // chunk's ext and offset-bit loads below, in one memory round trip.
+    ring_flush(ring, o, fl_a, fl_e);
+    fl_a = fl_e = 0;
+    wave_fence();
+    {
+      const uint32_t tn = min(t + c + lane, nt - 1u), gn = ng ? min(g2 + lane, ng - 1u) : 0u;
+      tv_n = tokb[tn];
+      sv_n = symb[tn];
+      gw_n = grpb[gn];
     }
     // lengths: L >= 15 has L-15 in 3 ext bytes, then M-4 >= 15 has M-19 in 2 (src/codec2.h:178-180); offsets: b
     // raw bits for symbol b (0..19), or one of the group before's last three (250..252). One scan carries the
@@ -664,31 +694,33 @@ __global__ __launch_bounds__(64) void k_dec_lz(const Block* __restrict__ blocks,
     const uint32_t ngc = readlane(rg, c - 1) + 1;
     if (on && (uint64_t)xq + nx > ne) bad = true;
     uint32_t L = tv >> 4, M = on ? (tv & 15u) + 4u : 0u;
-    if (!bad && (fl | fm)) {  // the token's (at most 5) ext bytes in one load (the decoder scratch goes on after them)
-      uint64_t w;
-      __builtin_memcpy(&w, extb + xq, 8);
-      if (fl) L += (uint32_t)w & 0xFFFFFFu;
-      if (fm) M += (uint32_t)(w >> (24u * fl)) & 0xFFFFu;
+    // the token's ext bytes (at most 5) and offset bits (at most 19), requested together by every lane at clamped
+    // places (no branch around the loads: one memory round trip for both), used below where they count. The ext
+    // load may read 8 bytes from ne (<= ECAP): the decoder scratch goes on after a block's ext bytes.
+    uint64_t xw;
+    __builtin_memcpy(&xw, extb + min(xq, ne), 8);
+    uint32_t bw = 0;
+    {
+      const uint32_t at = bq >> 3;
+      if (nob >= 4) {  // 4 bytes at a clamped address, shifted: bytes past the section read as 0
+        const uint32_t a = min(at, nob - 4u);
+        uint32_t v;
+        __builtin_memcpy(&v, bits + a, 4);
+        bw = at - a < 4u ? v >> (8u * (at - a)) : 0u;
+      } else {
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k)
+          if (at + k < nob) bw |= (uint32_t)bits[at + k] << (8 * k);
+      }
+    }
+    if (!bad && (fl | fm)) {
+      if (fl) L += (uint32_t)xw & 0xFFFFFFu;
+      if (fm) M += (uint32_t)(xw >> (24u * fl)) & 0xFFFFu;
     }
     uint32_t D = 0;
     if (on && !bad && !rep) {
-      if ((uint64_t)bq + nb_ > nbits_total) {
-        bad = true;
-      } else {
-        uint32_t w = 0;
-        const uint32_t at = bq >> 3;
-        if (nob >= 4) {  // 4 bytes at a clamped address, shifted: bytes past the section read as 0
-          const uint32_t a = min(at, nob - 4u);
-          uint32_t v;
-          __builtin_memcpy(&v, bits + a, 4);
-          w = at - a < 4u ? v >> (8u * (at - a)) : 0u;
-        } else {
-#pragma unroll
-          for (uint32_t k = 0; k < 4; ++k)
-            if (at + k < nob) w |= (uint32_t)bits[at + k] << (8 * k);
-        }
-        D = (1u << nb_) + ((w >> (bq & 7u)) & ((1u << nb_) - 1u));
-      }
+      if ((uint64_t)bq + nb_ > nbits_total) bad = true;
+      else D = (1u << nb_) + ((bw >> (bq & 7u)) & ((1u << nb_) - 1u));
     }
     // repeated offsets: the recent offsets of a group are those of the three tokens before its first one. So a
     // repeat j (symbol 250 + j) in a group that starts in this chunk at token gsk takes the offset of token
@@ -726,7 +758,7 @@ __global__ __launch_bounds__(64) void k_dec_lz(const Block* __restrict__ blocks,
     if (ballot(bad || gbad)) { fail = ballot(bad) ? E_TOKEN : E_GROUP; break; }
     const uint32_t cnt = on ? chunk_count(L, M, D) : 0u;
     const uint32_t cinc = scan_add(cnt), ccs = cinc - cnt;
-    sop[lane] = opk; sL[lane] = L; sM[lane] = M; sD[lane] = D; sli[lane] = lik; scs[lane] = ccs; srg[lane] = rg;
+    stk[lane] = TokF{opk, L, M, D, lik, ccs, rg, 0u};
     wave_fence();
     if (grem2 && nstart) gs_open = readlane(opk, readlane(gbeg, nstart - 1));  // (the open group's first token)
     {  // recent offsets carried on: those of the group that stays open (when it started here), then l0..l2
@@ -753,16 +785,15 @@ __global__ __launch_bounds__(64) void k_dec_lz(const Block* __restrict__ blocks,
     const uint32_t op0 = op_end - readlane(itot, 63);  // (the chunk's output: [op0, op_end))
     if (op_end - op0 <= RING - 16u) {
       const uint32_t lo = max(op_end > RING ? op_end - RING : 0u, ring_lo);
-      ring_copy(ring, o, litb, raw, lo, readlane(cinc, 63), on, ccs, sop, sL, sM, sD, sli, scs, srg, sown);
-      wave_fence();
-      ring_flush(ring, o, op0, op_end);
+      ring_copy(ring, o, litb, raw, lo, readlane(cinc, 63), on, ccs, stk, sown);
+      fl_a = op0;  // (to global memory with the next chunk, or after the loop)
+      fl_e = op_end;
     } else {  // a long chunk: group by group through global memory, and the window starts again after it
       for (uint32_t q = 0; q < ngc; ++q) {
         const uint64_t qm = ballot(on && rg == q);
         if (!qm) continue;
         const uint32_t ka = (uint32_t)__builtin_ctzll(qm), kb = 64u - (uint32_t)__builtin_clzll(qm);
-        group_copy(o, litb, raw, on && rg == q, ccs, readlane(ccs, ka), readlane(cinc, kb - 1), sop, sL, sM, sD, sli,
-                   scs, sown);
+        group_copy(o, litb, raw, on && rg == q, ccs, readlane(ccs, ka), readlane(cinc, kb - 1), stk, sown);
         wave_fence();  // this group's output is read by the next
       }
       ring_lo = op_end;
@@ -771,6 +802,8 @@ __global__ __launch_bounds__(64) void k_dec_lz(const Block* __restrict__ blocks,
   }
   if (!fail && (g != ng || grem != 0 || op + (nl - li) != raw)) fail = E_TOKEN;
   if (fail) { if (lane == 0) atomicOr(&err[b], fail); return; }
+  wave_fence();
+  ring_flush(ring, o, fl_a, fl_e);
   wave_copy(o + op, litb + li, nl - li);
 }
 

@@ -100,6 +100,12 @@ __device__ inline void wave_fence() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// waits until every vector memory operation of the wavefront has completed (s_waitcnt vmcnt(0); expcnt and lgkmcnt
+// at their maximum, so they do not wait). Placed before a loop whose body keeps loads in flight across iterations:
+// the compiler's wait at the loop head then no longer has to cover loads issued before the loop, which would make it
+// wait for the iteration's own in-flight load as well (the counter completes in issue order).
+__device__ inline void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // ---- byte-level helpers
 __device__ inline uint32_t ld32u(const uint8_t* p) {  // 4 bytes at any address, little endian (one unaligned load)
   uint32_t v;
