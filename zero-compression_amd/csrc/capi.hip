@@ -152,15 +152,15 @@ EncLayout enc_layout(uint32_t nb, uint32_t blog, uint32_t hbits) {
   L.qlarge = o; o += al((size_t)L.qcap * 8);
   L.qcnt = o; o += al(16);
   L.mark = o; o += al(N / 8);
-  // (the five symbol streams keep 256 bytes of slack: k_enc_rans reads them 256 bytes at a time)
-  L.lit = o; o += al(N + 256);
+  // (the five symbol streams keep 1 KiB of slack: k_enc_rans reads them 1 KiB at a time)
+  L.lit = o; o += al(N + 1024);
   L.LB = o; o += al((size_t)nb * G.TCAP * 4);
   L.TM = o; o += al((size_t)nb * G.TCAP * 4);
   L.TD = o; o += al((size_t)nb * G.TCAP * 4);
-  L.tok = o; o += al((size_t)nb * G.TCAP + 256);
-  L.sym = o; o += al((size_t)nb * G.TCAP + 256);
-  L.grp = o; o += al((size_t)nb * G.TCAP + 256);
-  L.ext = o; o += al((size_t)nb * G.ECAP + 256);
+  L.tok = o; o += al((size_t)nb * G.TCAP + 1024);
+  L.sym = o; o += al((size_t)nb * G.TCAP + 1024);
+  L.grp = o; o += al((size_t)nb * G.TCAP + 1024);
+  L.ext = o; o += al((size_t)nb * G.ECAP + 1024);
   L.offw = o; o += al((size_t)nb * (G.OCAP / 4 + 1) * 4);
   L.cnt = o; o += al((size_t)nb * 8 * 4);
   L.lanes = o; o += al((size_t)nb * lanes_per_block(G));

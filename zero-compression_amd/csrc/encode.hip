@@ -949,7 +949,26 @@ __global__ __launch_bounds__(64) void k_enc_rans(uint32_t blog, TokBufs T, RansB
   }
   for (uint32_t i = lane; i < 256; i += 64) hist[i] = 0;
   __syncthreads();
-  for (uint32_t i = lane; i < n; i += 64) atomicAdd(&hist[sy[i]], 1u);
+  {  // 16 bytes per lane per round of 1 KiB, the loads of 4 rounds requested together; the last partial round
+     // byte by byte (loads at any alignment: the ext stream's start is not 16-byte aligned)
+    const uint32_t nfull = n & ~1023u;
+    for (uint32_t base = 0; base < nfull; base += 4096u) {
+      uint32_t v[4][4];
+#pragma unroll
+      for (uint32_t k = 0; k < 4; ++k)
+        if (base + 1024u * k < nfull) __builtin_memcpy(v[k], sy + base + 1024u * k + 16u * lane, 16);
+#pragma unroll
+      for (uint32_t k = 0; k < 4; ++k) {
+        if (base + 1024u * k < nfull) {
+#pragma unroll
+          for (uint32_t w = 0; w < 4; ++w)
+#pragma unroll
+            for (uint32_t by = 0; by < 4; ++by) atomicAdd(&hist[(v[k][w] >> (8u * by)) & 255u], 1u);
+        }
+      }
+    }
+    for (uint32_t i = nfull + lane; i < n; i += 64) atomicAdd(&hist[sy[i]], 1u);
+  }
   __syncthreads();
   // 12-bit frequencies exactly as zb::normalize (src/codec.h:41-66), lane owns symbols 4*lane..4*lane+3
   uint32_t f[4], sum = 0, npres = 0;
@@ -1013,13 +1032,15 @@ __global__ __launch_bounds__(64) void k_enc_rans(uint32_t blog, TokBufs T, RansB
   }
   __syncthreads();
   // lane j encodes symbols j, j+64, ... from the last one back into its scratch, backwards (src/codec.h:112-126).
-  // Steps go 4 at a time: the 256 symbols of 4 steps arrive in LDS from one coalesced load made a group earlier,
-  // and the (at most 8) bytes the lane emits in them go, as one 8-byte write ending where its bytes stand (the bytes
-  // below them are garbage that the next group's write covers), into the lane's ring of 128 bytes in LDS. Every 4
-  // groups (at most 32 bytes) a lane whose next aligned 64-byte piece is complete stores it whole: no partial line
-  // of the scratch goes to memory, and most groups make one load and no store. (Between two such points the lane
-  // holds at most 63 + 32 bytes not yet stored, and writes 8 below them: the ring never overtakes them.)
-  __shared__ uint32_t stage[64];
+  // Steps go 4 at a time (a group). The 1024 symbols of 4 groups arrive in LDS from one coalesced 16-byte load per
+  // lane made 4 groups earlier (the symbol streams keep 1 KiB of slack for it). The (at most 8) bytes the lane emits
+  // in a group go, as one 8-byte write ending where its bytes stand (the bytes below them are garbage that the next
+  // group's write covers), into the lane's ring of 128 bytes in LDS. Every 4 groups (at most 32 bytes) a lane whose
+  // next aligned 64-byte piece is complete stores it whole: no partial line of the scratch goes to memory. Those
+  // stores are made at the start of the next 4 groups, after its symbols are in LDS and before its load, so that a
+  // wait for a load never waits for stores issued after it. (Between two such points the lane holds at most 63 + 32
+  // bytes not yet stored, and writes 8 below them: the ring never overtakes them.)
+  __shared__ uint32_t stage[256];
   __shared__ __align__(16) uint8_t rings[64 * RING_STRIDE];
   uint8_t* rl = rings + lane * RING_STRIDE;  // byte at address a: rl[(a - end) mod 128], with 8 bytes of spill past 128
   const uint32_t LC = lane_cap(G, s);
@@ -1030,53 +1051,66 @@ __global__ __launch_bounds__(64) void k_enc_rans(uint32_t blog, TokBufs T, RansB
   uint32_t fro = 64;     // ring offset of fl - 64
   uint32_t x = 1u << 23;
   const uint32_t NT = (n + 63u) / 64u;                 // steps: symbol lane + 64 t at step t, t from NT-1 down
-  int32_t j = (int32_t)((NT - 1u) >> 2);               // groups of steps [4j, 4j + 3]
-  uint32_t nxt;
-  __builtin_memcpy(&nxt, sy + 256u * (uint32_t)j + 4u * lane, 4);
-  for (; j >= 0; --j) {
+  int32_t J = (int32_t)((NT - 1u) >> 4);               // groups [4J, 4J + 3], steps [16J, 16J + 15]
+  typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+  v4u nxt;
+  __builtin_memcpy(&nxt, sy + 1024u * (uint32_t)J + 16u * lane, 16);
+  bool pend = false;  // a piece of the last 4 groups is to be stored
+  for (; J >= 0; --J) {
     wave_fence();
-    stage[lane] = nxt;
+    __builtin_memcpy(stage + 4u * lane, &nxt, 16);
     wave_fence();
-    if (j > 0) __builtin_memcpy(&nxt, sy + 256u * (uint32_t)(j - 1) + 4u * lane, 4);
-    uint64_t acc = 0;
-    uint32_t k = 0;  // bytes emitted in this group, the first at the top of acc
-    const uint8_t* st8 = (const uint8_t*)stage;
-#pragma unroll
-    for (int q = 3; q >= 0; --q) {
-      const uint32_t t = 4u * (uint32_t)j + (uint32_t)q;
-      const bool valid = t < NT && lane + 64u * t < n;
-      const uint32_t sym = st8[64u * (uint32_t)q + lane];
-      const uint32_t e = fc[sym];
-      const uint32_t fq = e & 0xFFFFu;
-      const uint32_t xmax = fq << 19;
-      // at most two bytes leave (x < 2^31, and x < 2^23 < ... after one; fq = 1 may take a second)
-      const bool e1 = valid && x >= xmax;
-      const uint32_t x1 = e1 ? x >> 8 : x;
-      const bool e2 = e1 && x1 >= xmax;
-      const uint32_t x2 = e2 ? x1 >> 8 : x1;
-      acc |= e1 ? (uint64_t)(x & 255u) << (8u * (7u - k)) : 0ull;
-      k += e1 ? 1u : 0u;
-      acc |= e2 ? (uint64_t)(x1 & 255u) << (8u * (7u - k)) : 0ull;
-      k += e2 ? 1u : 0u;
-      // exact x / fq for x < 2^31: floor(x * ceil(2^(31+s)/fq) / 2^(31+s)) with 2^s >= fq
-      const uint32_t sh = fq > 1 ? 32u - (uint32_t)__builtin_clz(fq - 1) : 0u;
-      const uint32_t qt = (uint32_t)(((uint64_t)x2 * rcp[sym]) >> (31 + sh));
-      const uint32_t xn = (qt << 12) + (x2 - qt * fq) + (e >> 16);
-      x = valid ? xn : x;
-    }
-    __builtin_memcpy(rl + ro, &acc, 8);
-    if (ro > 120u) {  // (the part past the ring's end belongs at its start)
-      const uint32_t over = ro - 120u;  // bytes past 128: 1..7
-      const uint64_t hi = acc >> (8u * (8u - over));
-      for (uint32_t i = 0; i < over; ++i) rl[i] = (uint8_t)(hi >> (8u * i));
-    }
-    ptr -= k;
-    ro = (ro - k) & 127u;
-    if ((j & 3) == 0 && ptr + 64 <= fl) {  // the piece [fl - 64, fl) is final
+    if (pend) {
       copy64(fl - 64, rl + fro);
       fl -= 64;
       fro = (fro - 64u) & 127u;
     }
+    __builtin_memcpy(&nxt, sy + 1024u * (uint32_t)(J > 0 ? J - 1 : 0) + 16u * lane, 16);  // (no branch: at J = 0
+                                                                                         // an unused reload)
+    const uint8_t* st8 = (const uint8_t*)stage;
+#pragma unroll 1
+    for (int32_t jj = 3; jj >= 0; --jj) {
+      const int32_t j = 4 * J + jj;  // (groups past the last step change nothing: no step of theirs is valid)
+      uint64_t acc = 0;
+      uint32_t k = 0;  // bytes emitted in this group, the first at the top of acc
+#pragma unroll
+      for (int q = 3; q >= 0; --q) {
+        const uint32_t t = 4u * (uint32_t)j + (uint32_t)q;
+        const bool valid = t < NT && lane + 64u * t < n;
+        const uint32_t sym = st8[256u * (uint32_t)jj + 64u * (uint32_t)q + lane];
+        const uint32_t e = fc[sym];
+        const uint32_t fq = e & 0xFFFFu;
+        const uint32_t xmax = fq << 19;
+        // at most two bytes leave (x < 2^31, and x < 2^23 < ... after one; fq = 1 may take a second)
+        const bool e1 = valid && x >= xmax;
+        const uint32_t x1 = e1 ? x >> 8 : x;
+        const bool e2 = e1 && x1 >= xmax;
+        const uint32_t x2 = e2 ? x1 >> 8 : x1;
+        acc |= e1 ? (uint64_t)(x & 255u) << (8u * (7u - k)) : 0ull;
+        k += e1 ? 1u : 0u;
+        acc |= e2 ? (uint64_t)(x1 & 255u) << (8u * (7u - k)) : 0ull;
+        k += e2 ? 1u : 0u;
+        // exact x / fq for x < 2^31: floor(x * ceil(2^(31+s)/fq) / 2^(31+s)) with 2^s >= fq
+        const uint32_t sh = fq > 1 ? 32u - (uint32_t)__builtin_clz(fq - 1) : 0u;
+        const uint32_t qt = (uint32_t)(((uint64_t)x2 * rcp[sym]) >> (31 + sh));
+        const uint32_t xn = (qt << 12) + (x2 - qt * fq) + (e >> 16);
+        x = valid ? xn : x;
+      }
+      __builtin_memcpy(rl + ro, &acc, 8);
+      if (ro > 120u) {  // (the part past the ring's end belongs at its start)
+        const uint32_t over = ro - 120u;  // bytes past 128: 1..7
+        const uint64_t hi = acc >> (8u * (8u - over));
+        for (uint32_t i = 0; i < over; ++i) rl[i] = (uint8_t)(hi >> (8u * i));
+      }
+      ptr -= k;
+      ro = (ro - k) & 127u;
+    }
+    pend = ptr + 64 <= fl;  // the piece [fl - 64, fl) is final
+  }
+  if (pend) {
+    copy64(fl - 64, rl + fro);
+    fl -= 64;
+    fro = (fro - 64u) & 127u;
   }
   ptr -= 4;
   ro = (ro - 4u) & 127u;  // (the ring offset of ptr - 8 again)
