@@ -132,6 +132,35 @@ __device__ inline uint64_t window8(uint32_t c0, uint32_t c1, uint32_t c2, uint32
   return (uint64_t)__builtin_bswap32(lo) << 32 | __builtin_bswap32(hi);
 }
 
+// 8 rANS steps of a lane from the 16-byte window c0..c3 (8 steps take at most 16 bytes: 4 from its first 8 bytes,
+// 4 from the 8 after what those took); the symbols go to stg[64 k + lane]. In the last steps of a stream (TAIL), a
+// lane whose symbol index first + 64 k + lane is past n takes no bytes. Returns the bytes taken.
+template <bool TAIL>
+__device__ inline uint32_t steps8(const uint8_t* slot, const uint32_t* fc, uint8_t* stg, uint32_t c0, uint32_t c1,
+                                  uint32_t c2, uint32_t c3, uint32_t& x, uint32_t first, uint32_t n) {
+  const uint32_t lane = lane_id();
+  uint32_t ob = 0;
+  uint64_t w = window8(c0, c1, c2, c3, 0u);
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k)
+    rans_step(slot, fc, stg + k * 64 + lane, !TAIL || first + k * 64 + lane < n, w, x, ob);
+  w = window8(c0, c1, c2, c3, ob >> 3);
+#pragma unroll
+  for (uint32_t k = 4; k < 8; ++k)
+    rans_step(slot, fc, stg + k * 64 + lane, !TAIL || first + k * 64 + lane < n, w, x, ob);
+  return ob >> 3;
+}
+// the 512 symbols of 8 steps from stg to d, 8 bytes per lane (every lane stores: no branch around the store); the
+// fences keep the LDS writes before and after it in order
+__device__ inline void store512(uint8_t* d, const uint8_t* stg) {
+  const uint32_t lane = lane_id();
+  wave_fence();
+  uint64_t v;
+  __builtin_memcpy(&v, stg + lane * 8, 8);
+  __builtin_memcpy(d + lane * 8, &v, 8);
+  wave_fence();
+}
+
 // ---------------------------------------------------------------- the streams: one wavefront per (block, stream)
 __global__ __launch_bounds__(64) void k_dec_streams(const Block* __restrict__ blocks, const uint8_t* __restrict__ comp,
                                                     uint32_t blog, uint8_t* __restrict__ lit, uint8_t* __restrict__ tok,
@@ -220,62 +249,68 @@ __global__ __launch_bounds__(64) void k_dec_streams(const Block* __restrict__ bl
   }
   __syncthreads();
   // 64 lanes decode side by side: lane j holds symbols j, j+64, ... A step needs 0, 1 or 2 more bytes of the lane
-  // (x >= 2^11 after the step: 2^15 <= x < 2^23 takes one, x < 2^15 two). Steps go 8 at a time over a 16-byte window
-  // of the lane's next bytes (8 steps take at most 16); the 16 bytes after the window were requested when the window
-  // was set, 8 steps earlier, so no step waits on memory. The symbols collect in LDS and leave 512 at a time in
-  // 16-byte stores (the vector memory counter is shared by loads and stores: a byte store per step would make every
-  // wait for a load also wait for the stores). Reads stop at the payload's end; a lane that consumes more bytes than
-  // its length is damaged.
+  // (x >= 2^11 after the step: 2^15 <= x < 2^23 takes one, x < 2^15 two). Steps go 16 at a time over a 32-byte
+  // window of the lane's next bytes (16 steps take at most 32), 8 over its first 16 bytes and 8 over the 16 after
+  // what those took; the 32 bytes after the window are requested when the 16 steps start and used when they end, so
+  // a load has 16 steps to arrive. The symbols collect in LDS and leave 512 at a time in 8-byte stores per lane (the
+  // vector memory counter is shared by loads and stores: a byte store per step would make every wait for a load also
+  // wait for the stores). Reads stop at the payload's end; a lane that consumes more bytes than its length is
+  // damaged.
   const uint8_t* lp = body + data0 + lend - ll;  // this lane's bytes
   const uint32_t lnl = ll;
   const uint8_t* lim = p + blk.comp_len;
   uint32_t x;
   __builtin_memcpy(&x, lp, 4);  // (ll >= 4: inside the lane)
-  uint32_t pos = 4;  // bytes of the lane consumed; the window c0..c3 holds bytes [pos, pos + 16)
-  uint32_t c0, c1, c2, c3, n0, n1, n2, n3, cd, nd;  // (n0..n3: the 16 bytes after the window, shifted by nd)
-  load16(lp, pos, lim, c0, c1, c2, c3, cd);
-  shift16(c0, c1, c2, c3, cd);
-  load16(lp, pos + 16, lim, n0, n1, n2, n3, nd);
-  wait_vm_all();  // (x, c and n are here: the step loop's head then waits for nothing from before the loop)
+  uint32_t pos = 4;  // bytes of the lane consumed; the window a0..a7 holds bytes [pos, pos + 32)
+  uint32_t a0, a1, a2, a3, a4, a5, a6, a7, ad0, ad1;
+  load16(lp, pos, lim, a0, a1, a2, a3, ad0);
+  load16(lp, pos + 16, lim, a4, a5, a6, a7, ad1);
+  shift16(a0, a1, a2, a3, ad0);
+  shift16(a4, a5, a6, a7, ad1);
+  wait_vm_all();  // (x and the window are here: the step loop's head then waits for nothing from before the loop)
   const uint32_t nsteps = (n + 63) >> 6, nfull = n >> 6;  // (every lane has a symbol in the first nfull steps)
   uint32_t b0 = 0;
-  for (; b0 + 8 <= nfull; b0 += 8) {
-    // 4 steps from the window's first 8 bytes (they take at most 8), 4 from the 8 after what those took
-    uint32_t ob = 0;
-    uint64_t w = window8(c0, c1, c2, c3, 0u);
-#pragma unroll
-    for (uint32_t k = 0; k < 4; ++k) rans_step(slot, fc, stg + k * 64 + lane, true, w, x, ob);
-    w = window8(c0, c1, c2, c3, ob >> 3);
-#pragma unroll
-    for (uint32_t k = 4; k < 8; ++k) rans_step(slot, fc, stg + k * 64 + lane, true, w, x, ob);
-    const uint32_t o = ob >> 3;
-    // the window moves on by o bytes: bytes [o, o + 16) of c0..c3 then n0..n3 (which arrived meanwhile; their load
-    // was moved back by nd bytes only near the payload's end)
-    if (ballot(nd != 0u)) shift16(n0, n1, n2, n3, nd);
-    realign16(c0, c1, c2, c3, n0, n1, n2, n3, o, c0, c1, c2, c3);
-    pos += o;
-    load16(lp, pos + 16, lim, n0, n1, n2, n3, nd);
-    // these 512 symbols to their place, after the load: a wait for that load then does not wait for these stores
-    wave_fence();
-    {  // (512 bytes, 8 per lane: every lane stores, so the loop has no branch around the store)
-      uint64_t v;
-      __builtin_memcpy(&v, stg + lane * 8, 8);
-      __builtin_memcpy(dest + b0 * 64 + lane * 8, &v, 8);
+  for (; b0 + 16 <= nfull; b0 += 16) {
+    // the 32 bytes after the window, requested now and used after 16 steps (16 steps take at most 32 bytes)
+    uint32_t p0, p1, p2, p3, p4, p5, p6, p7, pd0, pd1;
+    load16(lp, pos + 32, lim, p0, p1, p2, p3, pd0);
+    load16(lp, pos + 48, lim, p4, p5, p6, p7, pd1);
+    const uint32_t o1 = steps8<false>(slot, fc, stg, a0, a1, a2, a3, x, 0u, 0u);
+    uint32_t c0, c1, c2, c3;  // bytes [o1, o1 + 16) of the window (o1 <= 16)
+    realign16(a0, a1, a2, a3, a4, a5, a6, a7, o1, c0, c1, c2, c3);
+    store512(dest + b0 * 64, stg);  // (after the loads: a wait for them does not wait for this store)
+    const uint32_t o2 = steps8<false>(slot, fc, stg, c0, c1, c2, c3, x, 0u, 0u);
+    // the window moves on by o1 + o2 (<= 32) bytes: bytes [d, d + 32) of a0..a7 then p0..p7 (which arrived
+    // meanwhile; their loads were moved back by pd0 and pd1 bytes only near the payload's end)
+    if (ballot((pd0 | pd1) != 0u)) {
+      shift16(p0, p1, p2, p3, pd0);
+      shift16(p4, p5, p6, p7, pd1);
     }
-    wave_fence();
+    const uint32_t d = o1 + o2;
+    const bool hi = d > 16u;
+    const uint32_t e = hi ? d - 16u : d;
+    realign16(hi ? a4 : a0, hi ? a5 : a1, hi ? a6 : a2, hi ? a7 : a3, hi ? p0 : a4, hi ? p1 : a5, hi ? p2 : a6,
+              hi ? p3 : a7, e, c0, c1, c2, c3);
+    realign16(hi ? p0 : a4, hi ? p1 : a5, hi ? p2 : a6, hi ? p3 : a7, hi ? p4 : p0, hi ? p5 : p1, hi ? p6 : p2,
+              hi ? p7 : p3, e, a4, a5, a6, a7);
+    a0 = c0; a1 = c1; a2 = c2; a3 = c3;
+    pos += d;
+    store512(dest + (b0 + 8) * 64, stg);
   }
-  if (b0 < nsteps) {  // the last (at most 8) steps, within the window
-    uint32_t ob = 0;
-    uint64_t w = window8(c0, c1, c2, c3, 0u);
-#pragma unroll
-    for (uint32_t k = 0; k < 4; ++k) rans_step(slot, fc, stg + k * 64 + lane, (b0 + k) * 64 + lane < n, w, x, ob);
-    w = window8(c0, c1, c2, c3, ob >> 3);
-#pragma unroll
-    for (uint32_t k = 4; k < 8; ++k) rans_step(slot, fc, stg + k * 64 + lane, (b0 + k) * 64 + lane < n, w, x, ob);
-    pos += ob >> 3;
+  if (b0 < nsteps) {  // the last (at most 16) steps, within the window
+    const uint32_t o1 = steps8<true>(slot, fc, stg, a0, a1, a2, a3, x, b0 * 64, n);
+    pos += o1;
     wave_fence();
-    wave_copy_lds(dest + b0 * 64, stg, n - b0 * 64);
+    wave_copy_lds(dest + b0 * 64, stg, min(n - b0 * 64, 512u));
     wave_fence();
+    if (b0 + 8 < nsteps) {
+      uint32_t c0, c1, c2, c3;
+      realign16(a0, a1, a2, a3, a4, a5, a6, a7, o1, c0, c1, c2, c3);
+      pos += steps8<true>(slot, fc, stg, c0, c1, c2, c3, x, (b0 + 8) * 64, n);
+      wave_fence();
+      wave_copy_lds(dest + (b0 + 8) * 64, stg, n - (b0 + 8) * 64);
+      wave_fence();
+    }
   }
   if (ballot(pos > lnl) && lane == 0) atomicOr(&err[b], E_LANE);
 }
