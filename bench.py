@@ -336,7 +336,7 @@ def main():
     ap.add_argument("--workload", choices=["plain", "uneven", "serve", "sweep"], default="plain")
     ap.add_argument("--offered", type=float, default=20.0, help="serve workload: offered load in GB/s")
     ap.add_argument("--seconds", type=float, default=1.0, help="serve workload: arrival window")
-    ap.add_argument("--slots", type=int, default=3, help="serve workload: batches in flight")
+    ap.add_argument("--slots", type=int, default=6, help="serve workload: batches in flight")
     ap.add_argument("--files", type=int, default=200000, help="uneven workload: number of files")
     ap.add_argument("--sample-files", type=int, default=2000, help="uneven workload: files timed per-file")
     ap.add_argument("--streams", type=int, default=4, help="uneven workload: streams of the plan")

@@ -37,6 +37,36 @@ __global__ __launch_bounds__(256) void k_gather(const uint8_t* __restrict__ slab
   if (full + threadIdx.x < n) d[full + threadIdx.x] = s[full + threadIdx.x];
 }
 
+// one piece of a copy list: len bytes from src to dst. Either side may be pinned host memory, which the device reaches
+// over PCIe at the same address as the host (checked when it is registered with the server).
+struct CopyPiece {
+  uint64_t src, dst;
+  uint32_t len, pad;
+};
+constexpr uint32_t PIECE = 1u << 18;  // the largest piece: a request's bytes are cut into pieces of at most this
+// One workgroup per piece: 16 bytes per thread, 4 loads in flight per thread, then the tail (under 16 bytes) byte by
+// byte. A batch's requests thus move in one launch per direction instead of one copy call each.
+__global__ __launch_bounds__(256) void k_copy_list(const CopyPiece* __restrict__ pcs) {
+  const CopyPiece pc = pcs[blockIdx.x];
+  const uint8_t* s = (const uint8_t*)pc.src;
+  uint8_t* d = (uint8_t*)pc.dst;
+  const uint32_t n = pc.len, full = n & ~15u;
+  uint32_t i = threadIdx.x * 16;
+  for (; i + 3 * 256 * 16 < full; i += 4 * 256 * 16) {
+    uint4 v[4];
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) __builtin_memcpy(&v[k], s + i + k * 256 * 16, 16);
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) __builtin_memcpy(d + i + k * 256 * 16, &v[k], 16);
+  }
+  for (; i < full; i += 256 * 16) {
+    uint4 v;
+    __builtin_memcpy(&v, s + i, 16);
+    __builtin_memcpy(d + i, &v, 16);
+  }
+  if (full + threadIdx.x < n) d[full + threadIdx.x] = s[full + threadIdx.x];
+}
+
 double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -73,6 +103,9 @@ struct Slot {
   Block* h_tab = nullptr;
   uint32_t *h_len = nullptr, *h_err = nullptr;
   uint64_t *h_soff = nullptr, *h_goff = nullptr;
+  // copy lists: A holds a batch's inputs (phase 1) and later its compress payloads (phase 2), B its decompressed
+  // bytes (phase 1); pinned host tables and their device copies
+  CopyPiece *h_pa = nullptr, *h_pb = nullptr, *d_pa = nullptr, *d_pb = nullptr;
   uint32_t nenc = 0, ndec = 0;
 };
 
@@ -81,7 +114,8 @@ struct Server {
   uint32_t blog = 18;
   std::vector<uint32_t> cls;  // slab logs of the size classes: 10, 13, 16 (those below block_log), block_log
   uint64_t cap_in = 0, cap_slab = 0, cap_raw = 0, hbm = 0;
-  uint32_t cap_blk = 0, cap_dec = 0, job_blocks_log = 0;
+  uint32_t cap_blk = 0, cap_dec = 0, job_blocks_log = 0, cap_pieces = 0;
+  std::vector<std::pair<uint64_t, uint64_t>> host;  // pinned host ranges [a, e) that requests may read or write
   std::vector<Slot> slots;
   std::deque<SReq*> queue;
   uint64_t batches = 0, batched_reqs = 0, completed = 0;
@@ -107,6 +141,7 @@ struct Server {
     cap_raw = I;
     cap_blk = (uint32_t)std::min<uint64_t>(1u << 20, I / 1024 + (I >> blog) + 16);
     cap_dec = (uint32_t)std::min<uint64_t>(cap_blk, (I >> blog) + 1024);
+    cap_pieces = cap_blk + (uint32_t)(I / PIECE) + 16;  // (every request has a block; at most I bytes move per list)
     slots.resize(nslots);
     for (Slot& s : slots) {
       HIPCHK(g, hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking));
@@ -125,6 +160,10 @@ struct Server {
       HIPCHK(g, hipHostMalloc((void**)&s.h_err, (size_t)cap_blk * 4, hipHostMallocDefault));
       HIPCHK(g, hipHostMalloc((void**)&s.h_soff, (size_t)cap_blk * 8, hipHostMallocDefault));
       HIPCHK(g, hipHostMalloc((void**)&s.h_goff, (size_t)cap_blk * 8, hipHostMallocDefault));
+      HIPCHK(g, hipHostMalloc((void**)&s.h_pa, (size_t)cap_pieces * sizeof(CopyPiece), hipHostMallocDefault));
+      HIPCHK(g, hipHostMalloc((void**)&s.h_pb, (size_t)cap_pieces * sizeof(CopyPiece), hipHostMallocDefault));
+      HIPCHK(g, hipMalloc((void**)&s.d_pa, (size_t)cap_pieces * sizeof(CopyPiece)));
+      HIPCHK(g, hipMalloc((void**)&s.d_pb, (size_t)cap_pieces * sizeof(CopyPiece)));
       // the largest scratch any job of this slot can need, allocated now
       size_t ew = 0;
       for (uint32_t cl : cls) {
@@ -133,7 +172,8 @@ struct Server {
       }
       if (int rc = ensure(g, s.ework, ew)) return rc;
       if (int rc = ensure(g, s.dwork, dec_layout(cap_dec, blog).total)) return rc;
-      hbm += 2 * cap_in + cap_slab + cap_raw + (size_t)cap_blk * (sizeof(Block) + 24) + ew + s.dwork.n;
+      hbm += 2 * cap_in + cap_slab + cap_raw + (size_t)cap_blk * (sizeof(Block) + 24) + ew + s.dwork.n +
+             2 * (size_t)cap_pieces * sizeof(CopyPiece);
     }
     return ZGPU_OK;
   }
@@ -142,14 +182,47 @@ struct Server {
     for (Slot& s : slots) {
       (void)hipStreamSynchronize(s.st);
       for (void* p : {(void*)s.d_in, (void*)s.d_slab, (void*)s.d_pay, (void*)s.d_raw, (void*)s.d_tab,
-                      (void*)s.d_len, (void*)s.d_err, (void*)s.d_soff, (void*)s.d_goff, s.ework.p, s.dwork.p})
+                      (void*)s.d_len, (void*)s.d_err, (void*)s.d_soff, (void*)s.d_goff, (void*)s.d_pa,
+                      (void*)s.d_pb, s.ework.p, s.dwork.p})
         if (p) (void)hipFree(p);
-      for (void* p : {(void*)s.h_tab, (void*)s.h_len, (void*)s.h_err, (void*)s.h_soff, (void*)s.h_goff})
+      for (void* p : {(void*)s.h_tab, (void*)s.h_len, (void*)s.h_err, (void*)s.h_soff, (void*)s.h_goff,
+                      (void*)s.h_pa, (void*)s.h_pb})
         if (p) (void)hipHostFree(p);
       if (s.st) (void)hipStreamDestroy(s.st);
       if (s.ev) (void)hipEventDestroy(s.ev);
     }
     slots.clear();
+  }
+
+  // registers pinned host memory that requests read from or write to; the device must reach it at the host's address
+  int add_host(const void* p, uint64_t n) {
+    hipPointerAttribute_t a;
+    HIPCHK(g, hipPointerGetAttributes(&a, p));
+    if (a.type != hipMemoryTypeHost || a.devicePointer != p)
+      return fail(g, ZGPU_ERR_ARG, "serve: request buffers must be pinned host memory mapped at the same address");
+    host.push_back({(uint64_t)(uintptr_t)p, (uint64_t)(uintptr_t)p + n});
+    return ZGPU_OK;
+  }
+  bool host_ok(const void* p, uint64_t n) const {
+    const uint64_t a = (uint64_t)(uintptr_t)p;
+    for (const auto& h : host)
+      if (a >= h.first && a <= h.second && n <= h.second - a) return true;
+    return n == 0;
+  }
+  // appends the pieces of a copy of n bytes from src to dst
+  static uint32_t pieces(CopyPiece* pc, uint32_t np, const void* src, void* dst, uint64_t n) {
+    for (uint64_t o = 0; o < n; o += PIECE)
+      pc[np++] = CopyPiece{(uint64_t)(uintptr_t)src + o, (uint64_t)(uintptr_t)dst + o,
+                           (uint32_t)std::min<uint64_t>(PIECE, n - o), 0u};
+    return np;
+  }
+  // the copy list pc[0, np) as one launch on the slot's stream (its table goes up first)
+  int run_list(Slot& s, CopyPiece* h, CopyPiece* d, uint32_t np) {
+    if (!np) return ZGPU_OK;
+    HIPCHK(g, hipMemcpyAsync(d, h, (size_t)np * sizeof(CopyPiece), hipMemcpyHostToDevice, s.st));
+    k_copy_list<<<np, 256, 0, s.st>>>(d);
+    HIPCHK(g, hipGetLastError());
+    return ZGPU_OK;
   }
 
   // the size class of a block of raw bytes: the index of the smallest class slab that holds it
@@ -179,9 +252,10 @@ struct Server {
       const uint64_t sl = r->op == 0 ? slab_need(r) : 0;
       const uint64_t rw = r->op == 1 ? r->raw_len : 0;
       const uint64_t inl = (r->in_len + 255) & ~(uint64_t)255;
-      if (in + inl > cap_in || slab + sl > cap_slab || raw + rw > cap_raw || nenc + ndec + nbk > cap_blk ||
-          (r->op == 1 && ndec + nbk > cap_dec)) {
-        if (s.reqs.empty()) {  // (it can never fit: refused)
+      const bool bad_buf = !host_ok(r->in, r->in_len) || !host_ok(r->out, r->raw_len);
+      if (bad_buf || in + inl > cap_in || slab + sl > cap_slab || raw + rw > cap_raw ||
+          nenc + ndec + nbk > cap_blk || (r->op == 1 && ndec + nbk > cap_dec)) {
+        if (s.reqs.empty() || bad_buf) {  // (it can never fit, or its buffers are not the server's: refused)
           r->err = ZGPU_E_METHOD;
           r->t_done = now_s();
           ++completed;
@@ -215,9 +289,9 @@ struct Server {
       cbase[k] = cbase[k - 1] + ((uint64_t)count[k - 1] << cls[k - 1]);
     }
     std::vector<uint32_t> fill = at;
-    uint32_t di = nenc;
+    uint32_t di = nenc, npa = 0;
     for (SReq* r : s.reqs) {
-      if (r->in_len) HIPCHK(g, hipMemcpyAsync(s.d_in + r->ioff, r->in, r->in_len, hipMemcpyHostToDevice, s.st));
+      npa = pieces(s.h_pa, npa, r->in, s.d_in + r->ioff, r->in_len);
       if (r->op == 0) {
         for (uint64_t o = 0; o < r->raw_len; o += B) {
           const uint32_t n = (uint32_t)std::min(B, r->raw_len - o);
@@ -240,6 +314,7 @@ struct Server {
     s.nenc = nenc;
     s.ndec = ndec;
     const uint32_t nt = nenc + ndec;
+    if (int rc = run_list(s, s.h_pa, s.d_pa, npa)) return rc;  // the inputs, all in one launch
     HIPCHK(g, hipMemcpyAsync(s.d_tab, s.h_tab, (size_t)nt * sizeof(Block), hipMemcpyHostToDevice, s.st));
     if (nenc) HIPCHK(g, hipMemcpyAsync(s.d_soff, s.h_soff, (size_t)nenc * 8, hipMemcpyHostToDevice, s.st));
     // the encoder, class by class, in jobs of at most 2^28 slab positions
@@ -261,9 +336,10 @@ struct Server {
                               s.st, false))
         return rc;
       HIPCHK(g, hipMemcpyAsync(s.h_err + nenc, s.d_err + nenc, (size_t)ndec * 4, hipMemcpyDeviceToHost, s.st));
+      uint32_t npb = 0;
       for (SReq* r : s.reqs)
-        if (r->op == 1 && r->raw_len)
-          HIPCHK(g, hipMemcpyAsync(r->out, s.d_raw + r->roff, r->raw_len, hipMemcpyDeviceToHost, s.st));
+        if (r->op == 1) npb = pieces(s.h_pb, npb, s.d_raw + r->roff, r->out, r->raw_len);
+      if (int rc = run_list(s, s.h_pb, s.d_pb, npb)) return rc;  // the decompressed bytes, in one launch
     }
     if (nenc) HIPCHK(g, hipMemcpyAsync(s.h_len, s.d_len, (size_t)nenc * 4, hipMemcpyDeviceToHost, s.st));
     HIPCHK(g, hipEventRecord(s.ev, s.st));
@@ -303,9 +379,10 @@ struct Server {
     HIPCHK(g, hipMemcpyAsync(s.d_goff, s.h_goff, (size_t)s.nenc * 8, hipMemcpyHostToDevice, s.st));
     k_gather<<<s.nenc, 256, 0, s.st>>>(s.d_slab, s.d_soff, s.d_len, s.d_goff, s.d_pay);
     HIPCHK(g, hipGetLastError());
+    uint32_t npa = 0;  // (list A is free again: phase 1, whose table it held, has completed)
     for (SReq* r : s.reqs)
-      if (r->op == 0 && r->out_bytes)
-        HIPCHK(g, hipMemcpyAsync(r->out, s.d_pay + r->ioff, r->out_bytes, hipMemcpyDeviceToHost, s.st));
+      if (r->op == 0) npa = pieces(s.h_pa, npa, s.d_pay + r->ioff, r->out, r->out_bytes);
+    if (int rc = run_list(s, s.h_pa, s.d_pa, npa)) return rc;  // the payloads, in one launch
     HIPCHK(g, hipEventRecord(s.ev, s.st));
     s.phase = 2;
     return ZGPU_OK;
@@ -467,6 +544,12 @@ int zgpu_serve_bench(zgpu_t* g, const zgpu_serve_cfg_t* cfg, const uint8_t* lz, 
   // ---- serve
   Server srv;
   if ((rc = srv.init(g, cfg->hbm_budget, cfg->slots, blog))) { srv.release(); return rc; }
+  for (Pool& P : pools)
+    if (!rc) rc = srv.add_host(P.raw, P.n);
+  for (Pool& P : pools)
+    if (!rc) rc = srv.add_host(P.comp, P.coff.back() ? P.coff.back() : 1);
+  if (!rc) rc = srv.add_host(outs, out_total ? out_total : 1);
+  if (rc) { srv.release(); return rc; }
   size_t next = 0;
   const double t0 = now_s();
   for (SReq& r : reqs) r.t_arr += t0;
