@@ -531,7 +531,7 @@ def main():
         "encode_stage_ms_per_GiB": {k: round(v, 3) for k, v in enc_stages},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "device": props["name"],
+        "device": props["name"] or f"{props['arch']}, {props['cu_count']} CUs",  # (the name is empty on some boxes)
     }
     print(json.dumps(line), flush=True)
     gpu.close()
