@@ -167,7 +167,10 @@ __global__ __launch_bounds__(64) void k_dec_streams(const Block* __restrict__ bl
                                                     uint8_t* __restrict__ ext, uint8_t* __restrict__ sym,
                                                     uint8_t* __restrict__ grp, uint32_t* __restrict__ err) {
   const Geo G = geo_of(blog);
-  const uint32_t b = blockIdx.x / 5u, s = blockIdx.x % 5u;
+  // the literal streams of every block first, then the other four streams: a literal stream is usually many times
+  // longer than the others, and started first its wavefronts do not trail the kernel's end
+  const uint32_t nb = gridDim.x / 5u;
+  const uint32_t b = blockIdx.x % nb, s = blockIdx.x / nb;
   const uint32_t lane = threadIdx.x;
   const Block blk = blocks[b];
   if (blk.method != 1u) return;

@@ -938,7 +938,10 @@ __global__ __launch_bounds__(64) void k_enc_rans(uint32_t blog, TokBufs T, RansB
   __shared__ uint32_t fc[256];   // freq | cum << 16
   __shared__ uint32_t rcp[256];  // ceil(2^(31+s) / freq), s = ceil(log2 freq)
   const Geo G = geo_of(blog);
-  const uint32_t g = blockIdx.x, b = g / 5u, s = g % 5u, lane = threadIdx.x;
+  // the literal streams of every block first, then the other four streams (a literal stream is usually many times
+  // longer: started first, its wavefronts do not trail the kernel's end); g = 5 b + s indexes the outputs
+  const uint32_t nbk = gridDim.x / 5u;
+  const uint32_t b = blockIdx.x % nbk, s = blockIdx.x / nbk, g = 5u * b + s, lane = threadIdx.x;
   const uint32_t* cn = T.cnt + (size_t)b * 8;
   const uint32_t n = stream_count(cn, s);
   const uint8_t* sy = stream_syms(T, G, b, s);
