@@ -351,6 +351,14 @@ __device__ inline U128 shr128(const U128& v, uint32_t nbytes) {  // toward byte 
 }
 // 16 bytes of a pattern of period D < 16 whose first D bytes are in p, starting at period byte ph
 __device__ inline U128 pattern16(U128 e, uint32_t ph, uint32_t D) {
+  if ((D & (D - 1u)) == 0u) {  // a period of 1, 2, 4 or 8 bytes (runs, arrays of such values): its first D bytes
+                               // times a repeating constant (16 bytes are whole periods: the phase is 0)
+    const uint64_t v = D >= 8u ? e.lo
+                               : (e.lo & ((1ull << ((8u * D) & 63u)) - 1ull)) *
+                                     (D == 1u ? 0x0101010101010101ull : D == 2u ? 0x0001000100010001ull
+                                                                                : 0x0000000100000001ull);
+    return U128{v, v};
+  }
   for (uint32_t len = D; len < 16; len *= 2) e = splice(e, e, len);  // e[j] = period[j % D]
   // out[j] = e[ph + j] for j < 16 - ph, then e[ph + j - D] (one period back), which is byte j - (16 - ph) of e
   // shifted down by 16 - D
