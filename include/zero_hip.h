@@ -34,6 +34,7 @@ extern "C" {
 #define ZGPU_E_METHOD 0x80u
 
 #define ZGPU_STORED 0xFFFFFFFFu /* out_len of a block whose payload would not be smaller: write it stored */
+#define ZGPU_BAD_BLOCK 0xFFFFFFFEu /* out_len of a block table entry with raw_len > 2^block_log: nothing encoded */
 
 typedef struct zgpu zgpu_t;
 typedef void* zgpu_stream_t; /* a hipStream_t; NULL is the context's own stream */
@@ -95,10 +96,15 @@ int zgpu_event_elapsed_ms(zgpu_t* g, void* start, void* stop, float* ms);
  * 7-9 eight / 32). Scratch is the context's, grown on demand. */
 
 /* Encodes nb blocks (each raw_len <= 2^block_log). Block b's payload goes to out + (b << block_log) and its size to
- * out_len[b], or ZGPU_STORED when it would not be smaller than the block. */
+ * out_len[b], or ZGPU_STORED when it would not be smaller than the block. The kernels check the table themselves: an
+ * entry with raw_len > 2^block_log is read as an empty block and gets out_len ZGPU_BAD_BLOCK, and nothing is read
+ * or written outside its slab. */
 int zgpu_encode_blocks(zgpu_t* g, const uint8_t* d_in, const zgpu_block_t* d_blocks, uint32_t nb, uint32_t block_log,
                        int level, uint8_t* d_out, uint32_t* d_out_len, zgpu_stream_t s);
-/* Decodes nb blocks into out; err[b] receives ZGPU_E_* bits (0 = good). Asynchronous: read err after syncing. */
+/* Decodes nb blocks into out; err[b] receives ZGPU_E_* bits (0 = good). Asynchronous: read err after syncing. The
+ * kernels check the table themselves: raw_len > 2^block_log sets ZGPU_E_COUNTS before any byte of the block is read
+ * or written, and every other length comes from the payload and is checked before use (damaged input never makes
+ * the kernels access memory outside the block's payload, its scratch and its raw_len bytes at out + dst). */
 int zgpu_decode_blocks(zgpu_t* g, const uint8_t* d_comp, const zgpu_block_t* d_blocks, uint32_t nb, uint32_t block_log,
                        uint8_t* d_out, uint32_t* d_err, zgpu_stream_t s);
 /* crc[b] = CRC-32 of block b's raw bytes, at data + dst (use_dst != 0) or data + src (src/archive.h:376,618) */
@@ -121,6 +127,11 @@ int zgpu_compress(zgpu_t* g, const uint8_t* in, uint64_t n, uint32_t block_log, 
 int zgpu_decompress(zgpu_t* g, const uint8_t* comp, const uint32_t* comp_len, const uint8_t* method,
                     const uint32_t* raw_len, uint32_t nb, uint32_t block_log, uint8_t* out, uint32_t* err,
                     uint32_t* crc);
+/* One rANS stream section of n symbols (host buffers) by the block encoder's own kernels: histogram, 12-bit
+ * normalisation with its tie rules, 64 interleaved lanes, raw when not smaller (src/codec.h:41-147
+ * zb::normalize + zb::encode_stream, which it replaces). out needs n + 5 + 420 bytes; *out_len = the section's size.
+ * n <= 2^19. */
+int zgpu_encode_stream(zgpu_t* g, const uint8_t* sym, uint32_t n, uint8_t* out, uint32_t* out_len);
 /* crc[b] of the nb blocks of 2^block_log in data[0..n) (host buffer) */
 int zgpu_crc32(zgpu_t* g, const uint8_t* data, uint64_t n, uint32_t block_log, uint32_t* crc);
 /* crc[i] = CRC-32 of bytes [off[i], off[i] + len[i]) of the last zgpu_compress input (which = 0) or the last
@@ -168,9 +179,27 @@ int zgpu_plan_use_graphs(zgpu_plan_t* p, int on);
  * batching over `slots` batch slots, each with its own stream and preallocated device arenas), and reports latency
  * from arrival to the result in host memory. zgpu_serve_bench drives it with a synthetic open-loop load: Poisson
  * arrivals at offered_GBps of raw bytes for `seconds`, lognormal sizes, compress_share compress requests (level
- * uniform over 1 / 5 / 9), inputs cut from the two pools in random_share : 1 - random_share (decompress inputs are
- * the pools' blocks encoded at level 5 before the clock starts). After the run every response is round-tripped,
- * and alone_checks requests are served again one per batch and compared with their results in the mix. */
+ * uniform over 1 / 5 / 9), inputs cut from the two pools in random_share : 1 - random_share. A decompress request
+ * takes consecutive blocks of a pool as compressed before the clock starts: by the caller (dec_comp / dec_comp_len
+ * / dec_method: every 2^block_log block of the pool, back to back; method 0 stored, 1 ZB2), or when those are NULL
+ * by the GPU at level 5. After the run every response is round-tripped, and alone_checks requests are served again
+ * one per batch and compared with their results in the mix. With sample_every > 0, the responses of requests
+ * 0, sample_every, 2 sample_every, ... (in arrival order) are copied out, with what they were asked, until one of
+ * the three sample buffers is full. */
+typedef struct {
+  uint32_t op;          /* 0 compress, 1 decompress */
+  uint32_t level;       /* compress: 1..9 */
+  uint32_t pool;        /* 0 the LZ pool, 1 the random pool */
+  uint32_t nblk;        /* blocks of 2^block_log in the request */
+  uint64_t src_off;     /* where its raw bytes start in the pool (a block boundary for decompress) */
+  uint64_t raw_len;
+  uint64_t bytes_off;   /* its response in sample_bytes: compress, the payloads of its compressed blocks back to
+                           back; decompress, its raw bytes */
+  uint64_t bytes_len;
+  uint32_t blk_len_off; /* compress: its blocks' payload sizes (or ZGPU_STORED) at sample_blk_len[blk_len_off..] */
+  uint32_t err;         /* decompress: the ZGPU_E_* bits of its blocks */
+  double latency_ms;
+} zgpu_serve_sample_t;
 typedef struct {
   double offered_GBps, seconds, compress_share, random_share;
   double size_median, size_sigma;
@@ -180,6 +209,16 @@ typedef struct {
   int slots;
   uint64_t seed;
   uint32_t alone_checks;
+  const uint8_t* dec_comp[2];
+  const uint32_t* dec_comp_len[2];
+  const uint8_t* dec_method[2];
+  uint32_t sample_every;
+  uint32_t sample_cap;
+  zgpu_serve_sample_t* samples;
+  uint8_t* sample_bytes;
+  uint64_t sample_bytes_cap;
+  uint32_t* sample_blk_len;
+  uint32_t sample_blk_cap;
 } zgpu_serve_cfg_t;
 typedef struct {
   uint64_t requests, compress_requests, decompress_requests, bytes;
@@ -189,6 +228,7 @@ typedef struct {
   double mean_batch_requests;
   uint64_t hbm_bytes, slot_input_bytes;
   uint64_t mismatches, alone_checked, alone_mismatches;
+  uint64_t samples;
 } zgpu_serve_report_t;
 int zgpu_serve_bench(zgpu_t* g, const zgpu_serve_cfg_t* cfg, const uint8_t* lz_pool, uint64_t lz_len,
                      const uint8_t* random_pool, uint64_t random_len, zgpu_serve_report_t* rep);

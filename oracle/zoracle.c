@@ -419,7 +419,14 @@ done:
   return rc;
 }
 
-/* ------------------------------------------------------------------ the GPU encoder's model (kernels/zero.cl:44-317) */
+/* ------------------------------------------------------------------ the GPU encoder's model (kernels/zero.cl:44-317)
+ * kernels/zero.cl's algorithm with three departures, each one to make a block's output a function of its bytes:
+ *   - every bucket is sorted completely by (tag, position) (zero.cl scatters with atomics, :101, and sorts windows
+ *     of 16 entries stepping 12, :118-119, so its order within a bucket depends on timing);
+ *   - the hash of a position whose fifth byte lies past the block reads that byte as 0 (zero.cl reads the next
+ *     block's byte or stale slack, :55);
+ *   - buckets take min(block_log - 3, 15) hash bits (zero.cl: block_log - 3, i.e. 17 at 2^20); the tag is the
+ *     12 bits after them, as in zero.cl. */
 void zo_gpu_level(int level, uint32_t* ncand, uint32_t* lencap) {
   /* src/gpu_codec.h:66-70 */
   if (level <= 2) { *ncand = 2; *lencap = 16; }
@@ -451,28 +458,36 @@ void zo_gpu_match(const uint8_t* in, uint32_t n, uint32_t block_log, int level, 
   if (n < 4) return;
   const uint32_t keybits = hbits_of(block_log) + 12;
   const uint32_t count = n - 3;
-  /* every position ordered by (key, position): a position's candidates are the nearest earlier ones with its key */
+  /* every position ordered by (bucket, tag, position): zero.cl's srt with every bucket sorted completely */
   uint64_t* srt = (uint64_t*)malloc(sizeof(uint64_t) * count);
   for (uint32_t p = 0; p < count; ++p) srt[p] = (uint64_t)(hash5(in, n, p) >> (32 - keybits)) << 32 | p;
   qsort(srt, count, sizeof(uint64_t), cmp_u64);
   for (uint32_t s = 0; s < count; ++s) {
     const uint32_t p = (uint32_t)srt[s];
-    const uint32_t key = (uint32_t)(srt[s] >> 32);
+    const uint32_t tag = (uint32_t)(srt[s] >> 32) & 0xFFFu;
     const uint32_t maxlen = n - p < lencap ? n - p : lencap;
-    uint32_t best = 0, bestd = 0;
-    /* a run of one byte value first: it costs no offset bits (kernels/zero.cl:219-222) */
-    if (p > 0 && rd32(in + p - 1) == rd32(in + p)) {
-      best = 4 + prefix(in, p + 3, p + 4, maxlen - 4);
-      bestd = 1;
-    }
-    for (uint32_t k = 1; k <= ncand && best < maxlen; ++k) {
-      if (s < k || (uint32_t)(srt[s - k] >> 32) != key) break;
+    /* the candidates: of the ncand slots just before s, those with the same 12-bit tag (the key's low bits, the
+       bucket aside) and an earlier position (kernels/zero.cl:186-199) */
+    uint32_t cand[8], nc = 0;
+    for (uint32_t k = 1; k <= ncand; ++k) {
+      if (s < k) break;
       const uint32_t j = (uint32_t)srt[s - k];
-      if (rd32(in + j) != rd32(in + p)) continue;
-      if (best >= 4 && in[j + best] != in[p + best]) continue;
-      const uint32_t l = 4 + prefix(in, j + 4, p + 4, maxlen - 4);
-      const uint32_t d = p - j;
-      if (l > best || (l == best && d < bestd)) { best = l; bestd = d; }
+      if (((uint32_t)(srt[s - k] >> 32) & 0xFFFu) == tag && j < p) cand[nc++] = j;
+    }
+    uint32_t best = 0, bestd = 0;
+    if (nc) { /* only then (kernels/zero.cl:209): a run of one byte value first, it costs no offset bits (:219-222) */
+      if (p > 0 && rd32(in + p - 1) == rd32(in + p)) {
+        best = 4 + prefix(in, p + 3, p + 4, maxlen - 4);
+        bestd = 1;
+      }
+      for (uint32_t m = 0; m < nc && best < maxlen; ++m) {
+        const uint32_t j = cand[m];
+        if (rd32(in + j) != rd32(in + p)) continue;
+        if (best >= 4 && in[j + best] != in[p + best]) continue;
+        const uint32_t l = 4 + prefix(in, j + 4, p + 4, maxlen - 4);
+        const uint32_t d = p - j;
+        if (l > best || (l == best && d < bestd)) { best = l; bestd = d; }
+      }
     }
     match[p] = best >= 4 ? (best << 20) | (bestd - 1) : 0u;
   }

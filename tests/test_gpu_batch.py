@@ -115,3 +115,72 @@ def test_serving_loop_round_trips_and_mix_independence(gpu):
     assert rep["mismatches"] == 0
     assert rep["alone_checked"] == 24 and rep["alone_mismatches"] == 0
     assert rep["batches"] >= 2 and rep["p50_ms"] <= rep["p99_ms"] <= rep["max_ms"]
+
+
+def reference_pool(data, bl):
+    """every 2^bl block of data as the reference's CpuCodec writes it at level 5 (hash-chain depth 32,
+    src/main.cpp:54): the compiled reference when it is built, else the oracle (byte-identical to it)"""
+    from oracle_lib import Ref, ref_available
+    if ref_available():
+        return Ref().cpu_compress(data, bl, 32, 16)
+    B = 1 << bl
+    blob, clen, meth = [], [], []
+    for o in range(0, data.size, B):
+        blk = data[o: o + B].tobytes()
+        payload, st = O.cpu_compress_block(blk, 32)
+        blob.append(blk if st else payload)
+        clen.append(len(blob[-1]))
+        meth.append(0 if st else 1)
+    return np.frombuffer(b"".join(blob), dtype=np.uint8), np.array(clen, dtype=np.uint32), np.array(meth, dtype=np.uint8)
+
+
+def test_serving_at_c5_shape_against_the_oracle(gpu):
+    """The serving loop at the C5 shape: Poisson arrivals at 20 GB/s offered, 70 % compress, lognormal sizes (median
+    256 KiB, sigma 1.5, at most 256 MiB), LZ : random 9 : 1, levels 1 / 5 / 9, 2^18 blocks, a 64 GiB HBM budget, six
+    batch slots; a 0.5 s arrival window. The decompress requests decode blocks the reference's CPU codec wrote. Every
+    response is round-tripped by the server itself; a sample of them is checked here block by block: compress
+    responses equal the oracle's model of the GPU encoder at the request's level and decode under the reference's
+    decoder, decompress responses equal the pool's bytes."""
+    from oracle_lib import Ref, ref_available
+    bl, B = 18, 1 << 18
+    lz = pyzero.gen_lz(1, 256 << 20)
+    rnd = pyzero.gen_random(2, 64 << 20)
+    pools = [reference_pool(lz, bl), reference_pool(rnd, bl)]
+    assert (pools[1][2] == 0).all()  # (random bytes: every block stored)
+    rep = gpu.serve_bench(lz, rnd, offered_GBps=20.0, seconds=0.5, size_median=256 * 1024, size_sigma=1.5,
+                          size_max=256 << 20, hbm_budget=64 << 30, block_log=bl, slots=6, seed=23, alone_checks=32,
+                          dec_pools=pools, sample_every=53, sample_cap=256, sample_bytes=192 << 20)
+    print(f"C5 test: {rep['requests']} requests, sustained {rep['sustained_GBps']:.2f} GB/s, p50 {rep['p50_ms']:.1f} ms,"
+          f" p99 {rep['p99_ms']:.1f} ms, {rep['batches']} batches, {rep['samples']} samples", flush=True)
+    assert rep["requests"] > 5000
+    assert rep["mismatches"] == 0
+    assert rep["alone_checked"] == 32 and rep["alone_mismatches"] == 0
+    assert rep["p50_ms"] <= rep["p99_ms"] <= rep["max_ms"]
+    assert rep["sustained_GBps"] > 10.0  # (the server keeps up with most of the offered load even in a short window)
+    ref = Ref() if ref_available() else None
+    src = [lz, rnd]
+    seen = {0: 0, 1: 0}
+    levels = set()
+    for f, body, lens in rep["sample"]:
+        raw = src[f["pool"]][f["src_off"]: f["src_off"] + f["raw_len"]]
+        assert raw.size == f["raw_len"]
+        seen[f["op"]] += 1
+        if f["op"] == 1:
+            assert f["err"] == 0 and body == raw.tobytes()
+            continue
+        levels.add(f["level"])
+        offs = np.concatenate([[0], np.cumsum([0 if x == pyzero.STORED else int(x) for x in lens])])
+        assert offs[-1] == len(body)
+        nb = f["nblk"]
+        for k in sorted({0, nb - 1, nb // 3, (2 * nb) // 3}):  # (at most four blocks of a request: the model is slow)
+            blk = raw[k * B:(k + 1) * B].tobytes()
+            want, st = O.gpu_compress_block(blk, bl, f["level"])
+            if st:
+                assert lens[k] == pyzero.STORED
+                continue
+            got = body[offs[k]: offs[k + 1]]
+            assert got == want, (f, k)
+            if ref is not None:
+                rc, back = ref.decompress_block(got, len(blk))
+                assert rc == 0 and back == blk
+    assert seen[0] > 20 and seen[1] > 5 and levels == {1, 5, 9}, (seen, levels)

@@ -5,13 +5,46 @@ import zlib
 import numpy as np
 import pytest
 
+import hashlib
+import json
+import os
+import struct
+
 import pyzero
 from golden.corpus import corpus
-from oracle_lib import Oracle
+from golden.make_golden import stream_cases
+from oracle_lib import Oracle, Ref, ref_available
 
 pytestmark = pytest.mark.gpu
 O = Oracle()
+R = Ref() if ref_available() else None
 DEPTH = {1: 8, 5: 32, 9: 256}
+GOLD = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "golden.json")))
+
+
+def sections(payload):
+    """the five stream sections of a ZB2 payload with their symbol counts (src/codec2.h:140-151), and the offset bits"""
+    nt, nl, ne, nob, ng = struct.unpack_from("<5I", payload)
+    out, pos = [], 20
+    for cnt in (nl, nt, ne, nt, ng):
+        ln = struct.unpack_from("<I", payload, pos + 1)[0]
+        out.append((payload[pos: pos + 5 + ln], cnt))
+        pos += 5 + ln
+    assert len(payload) == pos + nob
+    return out
+
+
+def check_payload_against_reference(payload, blk):
+    """the reference CPU decoder gives blk back, and the reference's encode_stream writes every one of the payload's
+    five sections byte for byte from the symbols they hold"""
+    if R is None:
+        return
+    rc, back = R.decompress_block(payload, len(blk))
+    assert rc == 0 and back == blk
+    for sec, cnt in sections(payload):
+        r, syms = O.decode_stream(sec, cnt)
+        assert r == len(sec)
+        assert R.encode_stream(syms) == sec
 
 
 def split(d, bl):
@@ -39,9 +72,10 @@ def test_decode_reference_archives(gpu, bl):
             assert gpu.decompress(blocks, bl) == d, (name, bl, depth)
 
 
-@pytest.mark.parametrize("bl,level", [(16, 1), (16, 5), (18, 1), (18, 5), (18, 9), (20, 5), (20, 9)])
+@pytest.mark.parametrize("bl,level", [(16, 1), (16, 5), (16, 9), (18, 1), (18, 5), (18, 9), (20, 1), (20, 5), (20, 9)])
 def test_encode_equals_model(gpu, bl, level):
-    """the GPU encoder's payloads are byte-identical to the oracle's model of it, and decode (GPU and CPU)"""
+    """the GPU encoder's payloads are byte-identical to the oracle's model of it, decode under the oracle and the
+    reference's own decoder, and hold exactly the sections the reference's encode_stream writes for their symbols"""
     data = corpus(small=True)
     data["lz"] = pyzero.gen_lz(0, 3 << 18).tobytes()
     data["exec"] = pyzero.gen_exec(3, 3 << 18).tobytes()
@@ -60,6 +94,7 @@ def test_encode_equals_model(gpu, bl, level):
                 assert method == 1 and payload == want, (name, k, len(payload), len(want))
                 rc, back = O.decompress_block(payload, len(blk))
                 assert rc == 0 and back == blk
+                check_payload_against_reference(payload, blk)
         dec = gpu.decompress([(m, len(b), p if m else b) for b, (m, p) in zip(blocks, got)], bl)
         assert dec == d, name
 
@@ -144,31 +179,50 @@ def test_decode_repeat_chains(gpu, bl):
     assert gpu.decompress(blocks, bl) == want
 
 
-def test_damaged_payloads(gpu):
-    """damage is refused or decodes like the sequential decoder; it never faults"""
-    d = corpus(small=True)["mixed"][: 1 << 16]
-    payload, st = O.cpu_compress_block(d, 32)
-    assert not st
-    rng = random.Random(5)
+# the GPU's error bits for each way the sequential decoder (oracle, = src/codec2.h:155-203) refuses a payload: its
+# header (-1), its counts (-2), a stream section or the offset bits (-3, -4), the tokens (-5 .. -10). The GPU also
+# checks the counts against the block's caps (E_COUNTS) before the streams, and the stream decoder runs beside the
+# replay's checks, so a refusal may carry those bits too.
+STREAM_BITS = pyzero.E_STREAM | pyzero.E_TABLE | pyzero.E_LANE
+ALLOWED = {-1: pyzero.E_HDR, -2: pyzero.E_COUNTS, -3: STREAM_BITS | pyzero.E_COUNTS, -4: STREAM_BITS | pyzero.E_COUNTS}
+for _rc in range(-10, -4):
+    ALLOWED[_rc] = pyzero.E_TOKEN | pyzero.E_GROUP | pyzero.E_COUNTS
+
+
+@pytest.mark.parametrize("bl", [16, 18, 20])
+def test_damaged_payloads(gpu, bl):
+    """damage at 2^16 / 2^18 / 2^20 in payloads of the GPU encoder at levels 1, 5, 9 and of the CPU encoder: every
+    payload the sequential decoder refuses, the GPU refuses with the bits of that refusal; one it accepts decodes to
+    the same bytes, or is refused by the group rule alone; nothing faults"""
+    d = pyzero.gen_lz(7, 1 << bl).tobytes()
+    rng = random.Random(bl)
+    originals = [O.gpu_compress_block(d, bl, lv)[0] for lv in (1, 5, 9)] + [O.cpu_compress_block(d, 32)[0]]
     cases = []
-    for _ in range(300):
-        b = bytearray(payload)
-        for _ in range(rng.randrange(1, 8)):
-            b[rng.randrange(len(b))] = rng.randrange(256)
-        cases.append(bytes(b))
-    cases += [payload[:k] for k in (0, 5, 19, 20, 25, len(payload) // 2, len(payload) - 1)]
+    for payload in originals:
+        for _ in range(60):
+            b = bytearray(payload)
+            for _ in range(rng.randrange(1, 8)):
+                if rng.random() < 0.3:  # the header and section headers, where the counts and lengths are
+                    b[rng.randrange(min(len(b), 64))] = rng.randrange(256)
+                else:
+                    b[rng.randrange(len(b))] = rng.randrange(256)
+            cases.append(bytes(b))
+        cases += [payload[:k] for k in (0, 5, 19, 20, 25, len(payload) // 2, len(payload) - 1)]
     blocks = [(1, len(d), c) for c in cases]
     with pytest.raises(pyzero.DamagedError):
-        gpu.decompress(blocks, 16)
-    err = gpu.last_err
+        gpu.decompress(blocks, bl)
+    err = [int(e) for e in gpu.last_err]
+    refused_by_both = 0
     for k, c in enumerate(cases):
         rc, back = O.decompress_block(c, len(d))
         if rc != 0:
-            assert err[k] != 0, k  # the sequential decoder refuses it: so does the GPU
+            assert err[k] != 0 and err[k] & ~ALLOWED[rc] == 0, (k, rc, hex(err[k]))
+            refused_by_both += 1
         elif err[k] == 0:
-            assert gpu.decompress([(1, len(d), c)], 16) == back, k
+            assert gpu.decompress([(1, len(d), c)], bl) == back, k
         else:
-            assert err[k] == pyzero.ZGPU_ERR_DAMAGED or err[k] & 0x40, (k, err[k])  # only the group rule adds refusals
+            assert err[k] == pyzero.E_GROUP, (k, hex(err[k]))  # only the group rule adds refusals
+    assert refused_by_both > len(cases) // 4
 
 
 def test_stored_and_method_errors(gpu):
@@ -227,4 +281,106 @@ def test_large_roundtrip_device_resident(gpu):
     got = crc.download(dtype=np.uint32)
     assert got[0] == zlib.crc32(d[:B].tobytes()) and got[-1] == zlib.crc32(d[-B:].tobytes())
     for buf in (din, dtab, dout, dlen, dcomp, draw, derr, crc):
+        buf.free()
+
+
+def test_encode_stream_reference_bytes(gpu):
+    """the GPU's normaliser and 64-lane rANS encoder write the reference's stream sections byte for byte: on the
+    golden histograms that exercise normalize's excess and deficit tie rules (src/codec.h:41-66), on empty and
+    one-symbol streams, and on random histograms of every skew"""
+    cases = stream_cases()
+    assert len(cases) == len(GOLD["streams"])
+    for sym, v in zip(cases, GOLD["streams"]):
+        sec = gpu.encode_stream(sym)
+        assert len(sec) == v["section_len"] and hashlib.sha256(sec).hexdigest() == v["section_sha256"]
+    rng = np.random.default_rng(8)
+    extra = [b"", b"\x07", b"\x00" * 100000, bytes(range(256)) * 3]
+    for n in (1, 63, 64, 65, 1000, 4095, 4096, 4097, 100000, 1 << 19):
+        for a in (0.3, 1.0, 3.0):
+            extra.append(np.minimum(rng.zipf(1 + a, n) - 1, 255).astype(np.uint8).tobytes())
+        extra.append(rng.integers(0, 256, n, dtype=np.uint8).tobytes())
+    for sym in extra:
+        sec = gpu.encode_stream(sym)
+        assert sec == O.encode_stream(sym), len(sym)
+        if R is not None:
+            assert sec == R.encode_stream(sym), len(sym)
+
+
+@pytest.mark.parametrize("bl", [18, 20])
+def test_decode_format_maxima(gpu, bl):
+    """payloads at the format's limits decode like the sequential decoder: matches of 19 + 65535 bytes (the u16
+    length escape; the CPU encoder stops at 4096), literal runs longer than 2^16 (the u24 escape), overlapping
+    matches at periods 1..17 and far ones"""
+    n = 1 << bl
+    rng = random.Random(bl + 1)
+    out, toks = bytearray(rng.randbytes(40)), []
+    while True:
+        M = rng.choice([65554, 65554, 65535, 40000, 19, 20, 4097, 5000])
+        d = rng.choice([1, 2, 3, 4, 7, 8, 15, 16, 17, 1000, len(out)])
+        d = max(1, min(d, len(out)))
+        L = rng.choice([0, 1, 14, 15, 16, 300, 70000 if bl == 20 else 20000])
+        if len(out) + L + M > n:
+            break
+        out += rng.randbytes(L)
+        i = len(out)
+        for k in range(M):
+            out.append(out[i - d + k])
+        toks.append((i, M, d))
+    out += rng.randbytes(n - len(out))
+    out = bytes(out)
+    payload = O.pack_tokens(out, toks)
+    rc, back = O.decompress_block(payload, n)
+    assert rc == 0 and back == out
+    if R is not None:
+        rc, back = R.decompress_block(payload, n)
+        assert rc == 0 and back == out
+    assert max(m for _, m, _ in toks) == 65554
+    assert gpu.decompress([(1, n, payload)] * 3, bl) == out * 3
+
+
+def test_block_table_checks(gpu):
+    """device block tables are checked by the kernels: an entry with raw_len > 2^block_log is refused (E_COUNTS on
+    decode, ZGPU_BAD_BLOCK on encode) before any byte of it is touched, and the entries around it are unharmed.
+    The oversized entry comes first, so that its slab is followed by another block's and the buffers hold 2^bl
+    bytes more than the table names."""
+    bl = 16
+    B = 1 << bl
+    d = pyzero.gen_lz(3, 3 * B)
+    din = gpu.alloc(5 * B)
+    din.upload(d)
+    tab = pyzero.block_table([B + 1, B, B], src=np.array([0, B, 2 * B], dtype=np.uint64),
+                             dst=np.array([0, B, 2 * B], dtype=np.uint64))
+    dtab = gpu.alloc(tab.nbytes)
+    dtab.upload(tab)
+    dout = gpu.alloc(4 * B)
+    dlen = gpu.alloc(16)
+    gpu.encode_blocks(din.ptr, dtab.ptr, 3, bl, 5, dout.ptr, dlen.ptr)
+    gpu.sync()
+    lens = dlen.download(12, dtype=np.uint32)
+    slabs = dout.download()
+    assert lens[0] == pyzero.BAD_BLOCK
+    for b in (1, 2):
+        want, st = O.gpu_compress_block(d[b * B:(b + 1) * B].tobytes(), bl, 5)
+        assert not st and slabs[b * B: b * B + int(lens[b])].tobytes() == want
+    # decode: the first entry's raw_len is too large for both methods
+    good = [slabs[b * B: b * B + int(lens[b])].tobytes() for b in (1, 2)]
+    for method in (0, 1):
+        first = d[:B].tobytes() + b"x" if method == 0 else good[0]
+        blob = np.frombuffer(first + good[0] + good[1] + bytes(2 * B), dtype=np.uint8)
+        cl = [len(first), len(good[0]), len(good[1])]
+        t = pyzero.block_table([B + 1, B, B], comp_lens=cl, methods=[method, 1, 1],
+                               dst=np.array([0, B, 2 * B], dtype=np.uint64))
+        dcomp = gpu.alloc(blob.size)
+        dcomp.upload(blob)
+        dtab.upload(t)
+        draw = gpu.alloc(5 * B)
+        derr = gpu.alloc(16)
+        gpu.decode_blocks(dcomp.ptr, dtab.ptr, 3, bl, draw.ptr, derr.ptr)
+        gpu.sync()
+        err = derr.download(12, dtype=np.uint32)
+        assert err.tolist() == [pyzero.E_COUNTS, 0, 0], (method, err)
+        assert draw.download(2 * B, offset=B).tobytes() == d[B:].tobytes()
+        for buf in (dcomp, draw, derr):
+            buf.free()
+    for buf in (din, dtab, dout, dlen):
         buf.free()

@@ -129,45 +129,70 @@ def test_decompress_rejects_damage():
     assert O.decompress_block(payload[:10], len(d))[0] != 0
 
 
-# ---- the GPU encoder's model against a brute-force restatement (every earlier position scanned directly)
+# ---- the GPU encoder's model against a restatement of kernels/zero.cl's k_hash_count / k_bucket_scan / k_scatter /
+# k_sort / k_match in Python, with every bucket sorted completely (the determinism rule) and the byte past the block
+# read as 0
 def _hash5(d, p):
     v = d[p] | d[p + 1] << 8 | d[p + 2] << 16 | d[p + 3] << 24
     w = d[p + 4] if p + 4 < len(d) else 0
     return ((v * 2654435761) ^ (w * 0x9E3779B1 + (w >> 3))) & 0xFFFFFFFF
 
 
-def _brute_match(d, block_log, level):
-    ncand, lencap = (2, 16) if level <= 2 else (4, 16) if level <= 6 else (8, 32)
+def _zero_cl_srt(d, block_log):
+    """srt of one block as zero.cl lays it out: entries p | tag << 20 grouped by bucket (k_scatter,
+    kernels/zero.cl:92-103), each bucket sorted by entry value (k_sort's order, :108-144, over the whole bucket)."""
+    hbits = min(block_log - 3, 15)
     n = len(d)
-    keybits = min(block_log - 3, 15) + 12
-    out = [0] * n
-    if n < 4:
-        return out
-    count = n - 3
-    keys = [_hash5(d, p) >> (32 - keybits) for p in range(count)]
+    buckets = [[] for _ in range(1 << hbits)]
+    for p in range(n - 3 if n >= 4 else 0):  # positions with 4 bytes left (:64, :98)
+        hv = _hash5(d, p)
+        buckets[hv >> (32 - hbits)].append(p | (((hv >> (32 - hbits - 12)) & 0xFFF) << 20))
+    return [e for bk in buckets for e in sorted(bk)]
 
-    def common(a, b, lim):
+
+def _brute_match(d, block_log, level):
+    """k_match (kernels/zero.cl:172-236) over _zero_cl_srt, statement by statement."""
+    ncandl, lencap = (2, 16) if level <= 2 else (4, 16) if level <= 6 else (8, 32)
+    n = len(d)
+    out = [0] * n
+    sb = _zero_cl_srt(d, block_log)
+    count = len(sb)
+
+    def load4u(q):
+        return d[q] | d[q + 1] << 8 | d[q + 2] << 16 | d[q + 3] << 24
+
+    def match_len(a, b, maxlen):
         k = 0
-        while k < lim and d[a + k] == d[b + k]:
+        while k < maxlen and d[a + k] == d[b + k]:
             k += 1
         return k
 
-    for p in range(count):
-        maxlen = min(lencap, n - p)
+    for i in range(count):
+        e = sb[i]
+        p = e & 0xFFFFF
+        jc, any_ = [], False
+        for k in range(1, ncandl + 1):
+            c = sb[i - k] if i >= k else ~e & 0xFFFFFFFF
+            j = c & 0xFFFFF
+            ok = ((c ^ e) >> 20) == 0 and j < p
+            jc.append(j if ok else p)
+            any_ |= ok
         best, bestd = 0, 0
-        if p > 0 and d[p - 1: p + 3] == d[p: p + 4]:
-            best, bestd = 4 + common(p + 3, p + 4, maxlen - 4), 1
-        cands = [j for j in range(p - 1, -1, -1) if keys[j] == keys[p]][:ncand]
-        for j in cands:
-            if best >= maxlen:
-                break
-            if d[j: j + 4] != d[p: p + 4]:
-                continue
-            if best >= 4 and d[j + best] != d[p + best]:
-                continue
-            ln = 4 + common(j + 4, p + 4, maxlen - 4)
-            if ln > best or (ln == best and p - j < bestd):
-                best, bestd = ln, p - j
+        if any_:
+            pw = load4u(p)
+            maxlen = min(lencap, n - p)
+            if p > 0 and load4u(p - 1) == pw:
+                best, bestd = 4 + match_len(p + 3, p + 4, maxlen - 4), 1
+            for j in jc:
+                if best >= maxlen:
+                    break
+                if j == p or load4u(j) != pw:
+                    continue
+                if best >= 4 and d[j + best] != d[p + best]:
+                    continue
+                ln = 4 + match_len(j + 4, p + 4, maxlen - 4)
+                if ln > best or (ln == best and p - j < bestd):
+                    best, bestd = ln, p - j
         out[p] = (best << 20) | (bestd - 1) if best >= 4 else 0
     return out
 
@@ -211,9 +236,44 @@ def test_gpu_model_brute_force(level):
               (b"the cat sat on the mat. " * 20 + bytes(rng.randrange(256) for _ in range(300))) * 3,
               bytes(rng.choice(b"ab") for _ in range(900)) + bytes(200) + b"xyz" * 100]
     for d in inputs:
+        for block_log in (16, 18):
+            m = O.gpu_match(d, block_log, level)
+            assert m.tolist() == _brute_match(d, block_log, level)
+            assert O.gpu_tokens(d, block_log, level) == _brute_tokens(d, m.tolist(), level)
+
+
+def test_gpu_model_run_check_needs_a_candidate():
+    """zero.cl runs the distance-1 check only inside `if (any)` (kernels/zero.cl:209-222). At the end of a run of
+    one value (bytes p-1 .. p+3 equal, p+4 another), the run check alone gives a match of 4 at distance 1: it must
+    give nothing when no earlier slot holds the position's tag, and decide when the only candidates are tag
+    collisions whose bytes differ."""
+    rng = random.Random(21)
+    parts = []
+    for _ in range(7000):  # runs of exactly 5 followed by a random byte, among random bytes
+        q = rng.randrange(256)
+        parts.append(bytes([q]) * 5 + bytes([(q + 1 + rng.randrange(255)) % 256]))
+        parts.append(bytes(rng.randrange(256) for _ in range(rng.randrange(4))))
+    d = b"".join(parts)[: 1 << 16]
+    n = len(d)
+    sb = _zero_cl_srt(d, 16)
+    slot = {e & 0xFFFFF: i for i, e in enumerate(sb)}
+    seen = {"none": 0, "collision": 0}
+    for level in (1, 5, 9):
+        ncand = 2 if level <= 2 else 4 if level <= 6 else 8
         m = O.gpu_match(d, 16, level)
-        assert m.tolist() == _brute_match(d, 16, level)
-        assert O.gpu_tokens(d, 16, level) == _brute_tokens(d, m.tolist(), level)
+        for p in range(1, n - 4):
+            if not (d[p - 1: p + 3] == d[p: p + 4] and d[p + 4] != d[p + 3]):
+                continue
+            i = slot[p]
+            cands = [sb[i - k] & 0xFFFFF for k in range(1, ncand + 1)
+                     if i >= k and ((sb[i - k] ^ sb[i]) >> 20) == 0 and (sb[i - k] & 0xFFFFF) < p]
+            if not cands:
+                assert m[p] == 0, (level, p)
+                seen["none"] += 1
+            elif all(d[j: j + 4] != d[p: p + 4] for j in cands):
+                assert m[p] == 4 << 20, (level, p)
+                seen["collision"] += 1
+    assert seen["none"] > 100 and seen["collision"] > 0, seen
 
 
 def test_pack_back_half_matches_reference_encoder():

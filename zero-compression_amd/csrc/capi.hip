@@ -409,6 +409,50 @@ int zgpu_encode_blocks(zgpu_t* g, const uint8_t* d_in, const zgpu_block_t* d_blo
   return encode_run(g, d_in, (const Block*)d_blocks, nb, S, level, d_out, d_out_len, (uint8_t*)g->enc.p, st, true);
 }
 
+// one stream section (u8 kind, u32 len, body) of n symbols by the block encoder's own kernels: the symbols are the
+// literal stream of a single block whose other streams are empty, k_enc_rans normalises and encodes them, k_enc_pack
+// lays the payload out, and the section is the payload's first one. The block is 2^blog with 2^blog >= 2n + 4096,
+// so the payload always fits and is never "stored".
+int zgpu_encode_stream(zgpu_t* g, const uint8_t* sym, uint32_t n, uint8_t* out, uint32_t* out_len) {
+  if (n > (1u << 19)) return fail(g, ZGPU_ERR_ARG, "a stream holds at most 2^19 symbols");
+  uint32_t blog = 16;
+  while (blog < 20 && (1ull << blog) < 2ull * n + 4096) ++blog;
+  const Geo G = geo_of(blog);
+  const EncLayout L = enc_layout(1, blog, hbits_of(blog));
+  if (int rc = ensure(g, g->enc, L.total)) return rc;
+  if (int rc = ensure(g, g->aux, al(G.B) + al(64) + al(32))) return rc;
+  g->last_len[0] = g->last_len[1] = 0;
+  uint8_t* base = (uint8_t*)g->enc.p;
+  uint8_t* d_out = (uint8_t*)g->aux.p;
+  uint32_t* d_len = (uint32_t*)(d_out + al(G.B));
+  zgpu_block_t* d_tab = (zgpu_block_t*)((uint8_t*)d_len + al(64));
+  zgpu_block_t tab{};
+  tab.raw_len = G.B;
+  const uint32_t cnt[8] = {0, n, 0, 0, 0, 0, 0, 0};  // tokens, literals, ext bytes, offset-bit bytes, groups
+  hipStream_t st = g->stream;
+  TokBufs T{base + L.lit,  (uint32_t*)(base + L.LB), (uint32_t*)(base + L.TM), (uint32_t*)(base + L.TD),
+            base + L.tok,  base + L.sym,              base + L.grp,              base + L.ext,
+            (uint32_t*)(base + L.offw), (uint32_t*)(base + L.cnt)};
+  RansBufs R{base + L.lanes, (uint16_t*)(base + L.freq), (uint32_t*)(base + L.lanelen), (uint32_t*)(base + L.sinfo)};
+  if (n) HIPCHK(g, hipMemcpyAsync(T.lit, sym, n, hipMemcpyHostToDevice, st));
+  HIPCHK(g, hipMemcpyAsync(T.cnt, cnt, sizeof cnt, hipMemcpyHostToDevice, st));
+  HIPCHK(g, hipMemcpyAsync(d_tab, &tab, sizeof tab, hipMemcpyHostToDevice, st));
+  k_enc_rans<<<5, 64, 0, st>>>(blog, T, R);
+  k_enc_pack<<<1, 256, 0, st>>>((const Block*)d_tab, blog, T, R, d_out, d_len);
+  HIPCHK(g, hipGetLastError());
+  uint32_t total = 0;
+  HIPCHK(g, hipMemcpyAsync(&total, d_len, 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(g, hipStreamSynchronize(st));
+  if (total == ZGPU_STORED || total < 20 + 5 * 5) return fail(g, ZGPU_ERR_DEVICE, "stream encoder: impossible payload");
+  uint8_t hdr[5];
+  HIPCHK(g, hipMemcpy(hdr, d_out + 20, 5, hipMemcpyDeviceToHost));
+  const uint32_t len = (uint32_t)hdr[1] | (uint32_t)hdr[2] << 8 | (uint32_t)hdr[3] << 16 | (uint32_t)hdr[4] << 24;
+  if (20ull + 5 + len > total) return fail(g, ZGPU_ERR_DEVICE, "stream encoder: section past the payload");
+  HIPCHK(g, hipMemcpy(out, d_out + 20, 5 + (size_t)len, hipMemcpyDeviceToHost));
+  *out_len = 5 + len;
+  return ZGPU_OK;
+}
+
 // ---------------------------------------------------------------- decode
 static int decode_run(zgpu_t* g, const uint8_t* d_comp, const Block* blocks, uint32_t nb, uint32_t blog,
                       uint8_t* d_out, uint32_t* d_err, uint8_t* base, hipStream_t st, bool prof) {

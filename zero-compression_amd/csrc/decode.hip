@@ -35,6 +35,7 @@ struct Sections {
 // section `want` (0..4)
 __device__ inline Sections read_sections(const uint8_t* p, uint32_t clen, uint32_t raw, const Geo& G, uint32_t want) {
   Sections S = {};
+  if (raw > G.B) { S.err = E_COUNTS; return S; }  // (a block table entry larger than the block size)
   if (clen < 20) { S.err = E_HDR; return S; }
   S.nt = ld32u(p); S.nl = ld32u(p + 4); S.ne = ld32u(p + 8); S.nob = ld32u(p + 12); S.ng = ld32u(p + 16);
   if (S.nl > raw || S.nt > raw || (uint64_t)S.ne > 5ull * S.nt || S.ng > S.nt || S.nt > G.TCAP || S.ne > G.ECAP) {
@@ -663,6 +664,7 @@ __global__ __launch_bounds__(64) void k_dec_lz(const Block* __restrict__ blocks,
   uint8_t* o = out + blk.dst;
   const uint8_t* p = comp + blk.src;
   const uint32_t raw = blk.raw_len;
+  if (raw > G.B) { if (lane == 0) atomicOr(&err[b], E_COUNTS); return; }  // (before any access: see read_sections)
   if (blk.method == 0u) {
     if (blk.comp_len != raw) { if (lane == 0) atomicOr(&err[b], E_METHOD); return; }
     wave_copy(o, p, raw);

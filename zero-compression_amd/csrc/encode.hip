@@ -102,7 +102,7 @@ __global__ __launch_bounds__(1024) void k_enc_hash(const Block* __restrict__ blo
   const uint32_t HBITS = hbits, HB = 1u << HBITS;
   const Block blk = blocks[b];
   const uint8_t* inb = in + blk.src;
-  const uint32_t n = blk.raw_len;
+  const uint32_t n = enc_len(blk, blog);
   const uint32_t count = n >= 4 ? n - 3 : 0;
   for (uint32_t i = tid; i < HB; i += 1024) cnt[i] = 0;
   __syncthreads();
@@ -190,7 +190,7 @@ __global__ __launch_bounds__(1024) void k_enc_place(const Block* __restrict__ bl
   const uint32_t HB = 1u << hbits, pb = hbits - pbits, NB = 1u << pb;
   const Block blk = blocks[b];
   const uint8_t* inb = in + blk.src;
-  const uint32_t n = blk.raw_len;
+  const uint32_t n = enc_len(blk, blog);
   const uint32_t* bs = bstart + (size_t)b * (HB + 1);
   const uint32_t ps = bs[P << pb], pe = bs[(P + 1) << pb], m = pe - ps;
   uint32_t* lc = sm;
@@ -416,13 +416,15 @@ __global__ __launch_bounds__(1024) void k_enc_sort_l(uint32_t blog, uint32_t hbi
 }
 
 // ---------------------------------------------------------------- 3. match finder (one thread per sorted slot)
-// The best match of a position is among its run check (distance 1) and the ncand slots just before its own in the
-// sorted order that hold the same key (kernels/zero.cl:172-236): the longest, the nearest on a tie. A wavefront takes
-// 64 consecutive slots of one block. Each lane reads the first W bytes of its position once (W = 16 for a length cap
-// of 16, 32 for one of 32; bytes past the block read as 0) and lanes 0..7 also those of the 8 slots before the
+// The candidates of a position are the ncand slots just before its own in the sorted order that hold its 12-bit tag
+// (the key's low 12 bits) and an earlier position (kernels/zero.cl:186-199); a slot of another bucket with the same
+// tag counts too, as there. Only when one exists, the run check (distance 1) goes first (kernels/zero.cl:209-222),
+// then each candidate in slot order; the longest wins, the first found on a tie (zero.cl's early reject of a
+// candidate whose byte at the best length differs makes its nearer-on-a-tie rule unreachable). A wavefront takes 64
+// consecutive slots of one block. Each lane reads the first W bytes of its position once (W = 16 for a length cap of
+// 16, 32 for one of 32; bytes past the block read as 0) and lanes 0..7 also those of the 8 slots before the
 // wavefront's first; windows, positions and keys go to LDS, and each candidate is compared there, window against
-// window. The key of a candidate is its hash recomputed from its window, so that a different key (another bucket, or
-// the same bucket and another tag) ends the candidates as the sorted order's run of one key ends.
+// window. The key of a candidate is its hash recomputed from its window.
 constexpr uint32_t MX = 8;             // most candidates of any level
 constexpr uint32_t NOPOS = 0xFFFFFFFFu;  // an LDS entry with no slot behind it
 
@@ -495,27 +497,35 @@ __device__ inline void match_slots(const uint8_t* __restrict__ inb, uint32_t n, 
   }
   wave_fence();
   const uint32_t maxlen = min(lencap, n - p);
+  // which of the slots before this one are candidates: bit k - 1 for slot i - k
+  uint32_t okm = 0;
+#pragma unroll
+  for (uint32_t k = 1; k <= MX; ++k) {
+    if (k > ncand) break;  // (uniform)
+    const uint32_t idx = MX + lane - k;
+    const uint32_t cp = lpos[idx], ck = lkey[idx];
+    const bool ok = act && cp != NOPOS && ((ck ^ key) & 0xFFFu) == 0 && cp < p;
+    okm |= ok ? 1u << (k - 1) : 0u;
+  }
   uint32_t best = 0, bestd = 0;
-  if (run) {  // a run of one byte value (kernels/zero.cl:219-222): 5 equal bytes from p - 1
+  if (run && okm) {  // a run of one byte value (kernels/zero.cl:219-222): 5 equal bytes from p - 1
     uint32_t r[NW];
 #pragma unroll
     for (int q = 0; q < NW; ++q) r[q] = w[0];
     best = min(common_prefix<NW>(w, r), maxlen);
     bestd = 1;
   }
-  // every candidate compared, with selects rather than branches (lanes differ in how many share their key)
-  bool go = act;
+  // every candidate compared, with selects rather than branches (lanes differ in how many candidates they have)
 #pragma unroll
   for (uint32_t k = 1; k <= MX; ++k) {
     if (k > ncand) break;  // (uniform)
     const uint32_t idx = MX + lane - k;
-    const uint32_t cp = lpos[idx], ck = lkey[idx];
-    go = go && cp != NOPOS && ck == key;  // (sorted by key: no earlier slot has this key)
+    const uint32_t cp = lpos[idx];
     uint32_t c[NW];
 #pragma unroll
     for (int q = 0; q < NW; ++q) c[q] = lwin[idx * NW + q];
     const uint32_t l = min(common_prefix<NW>(w, c), maxlen);
-    const bool better = go && l >= 4 && l > best;
+    const bool better = ((okm >> (k - 1)) & 1u) && l >= 4 && l > best;
     best = better ? l : best;
     bestd = better ? p - cp : bestd;
   }
@@ -531,7 +541,7 @@ __global__ __launch_bounds__(256) void k_enc_match(const Block* __restrict__ blo
   const uint32_t b = wg >> (blog - 8);
   const uint32_t i0 = ((wg << 8) + (threadIdx.x & ~63u)) & ((1u << blog) - 1u);
   const Block blk = blocks[b];
-  const uint32_t n = blk.raw_len;
+  const uint32_t n = enc_len(blk, blog);
   if (i0 >= n) return;  // (uniform over the wavefront)
   if (blk.level) level_params(blk.level, ncand, lencap);
   const uint32_t count = n >= 4 ? n - 3 : 0;
@@ -638,7 +648,7 @@ __global__ __launch_bounds__(1024) void k_enc_parse(const Block* __restrict__ bl
   uint32_t* sout = sin + walkers;  // where that walk met walk q (| 1 << 31), or left the segment
   const Block blk = blocks[b];
   const uint8_t* inb = in + blk.src;
-  const uint32_t n = blk.raw_len;
+  const uint32_t n = enc_len(blk, blog);
   uint32_t* m = match + ((size_t)b << blog);
   if (blk.level) { uint32_t nc; level_params(blk.level, nc, lencap); }
   for (uint32_t i = k; i < B / 32u; i += T) bm[i] = 0;
@@ -749,7 +759,7 @@ __global__ __launch_bounds__(64) void k_enc_tokens(const Block* __restrict__ blo
   const uint32_t b = blockIdx.x, lane = threadIdx.x;
   const Block blk = blocks[b];
   const uint8_t* inb = in + blk.src;
-  const uint32_t n = blk.raw_len;
+  const uint32_t n = enc_len(blk, blog);
   const uint32_t* m = match + ((size_t)b << blog);
   if (blk.level) { uint32_t nc; level_params(blk.level, nc, lencap); }
   const uint32_t* mk = mark + ((size_t)b << (blog - 5));
@@ -1152,6 +1162,7 @@ __global__ __launch_bounds__(256) void k_enc_pack(const Block* __restrict__ bloc
   __shared__ uint32_t loff[65];
   const Geo G = geo_of(blog);
   const uint32_t b = blockIdx.x, tid = threadIdx.x;
+  if (blocks[b].raw_len > G.B) { if (tid == 0) outlen[b] = BAD_BLOCK; return; }
   const uint32_t n = blocks[b].raw_len;
   const uint32_t* cn = T.cnt + (size_t)b * 8;
   const uint32_t nob = cn[3];

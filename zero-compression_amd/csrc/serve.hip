@@ -452,10 +452,32 @@ int zgpu_serve_bench(zgpu_t* g, const zgpu_serve_cfg_t* cfg, const uint8_t* lz, 
     std::vector<uint32_t> clen;
     std::vector<uint8_t> meth;
   } pools[2] = {{lz, lz_len}, {rnd, rnd_len}};
-  for (Pool& P : pools) {
+  for (int pi = 0; pi < 2; ++pi) {
+    Pool& P = pools[pi];
     HIPCHK(g, hipHostMalloc((void**)&P.raw, P.n, hipHostMallocDefault));
     memcpy(P.raw, P.src, P.n);
     const uint32_t nb = (uint32_t)((P.n + B - 1) / B);
+    P.coff.resize(nb + 1);
+    P.clen.resize(nb);
+    P.meth.resize(nb);
+    if (cfg->dec_comp[pi]) {  // compressed by the caller
+      if (!cfg->dec_comp_len[pi] || !cfg->dec_method[pi])
+        return fail(g, ZGPU_ERR_ARG, "serve: dec_comp needs dec_comp_len and dec_method");
+      uint64_t t = 0;
+      for (uint32_t b = 0; b < nb; ++b) {
+        const uint32_t raw = (uint32_t)std::min<uint64_t>(B, P.n - (uint64_t)b * B);
+        P.meth[b] = cfg->dec_method[pi][b];
+        P.clen[b] = cfg->dec_comp_len[pi][b];
+        if (P.meth[b] > 1 || (P.meth[b] == 0 && P.clen[b] != raw))
+          return fail(g, ZGPU_ERR_ARG, "serve: a caller-compressed block has an impossible method or length");
+        P.coff[b] = t;
+        t += P.clen[b];
+      }
+      P.coff[nb] = t;
+      HIPCHK(g, hipHostMalloc((void**)&P.comp, t ? t : 1, hipHostMallocDefault));
+      memcpy(P.comp, cfg->dec_comp[pi], t);
+      continue;
+    }
     std::vector<uint32_t> len(nb);
     uint8_t* slabs = (uint8_t*)malloc((size_t)nb * B);
     if (!slabs) return fail(g, ZGPU_ERR_NOMEM, "serve: host memory");
@@ -465,9 +487,6 @@ int zgpu_serve_bench(zgpu_t* g, const zgpu_serve_cfg_t* cfg, const uint8_t* lz, 
       rc = zgpu_compress(g, P.raw + o, n, blog, 5, slabs + o, len.data() + b0, nullptr);
     }
     if (rc) { free(slabs); return rc; }
-    P.coff.resize(nb + 1);
-    P.clen.resize(nb);
-    P.meth.resize(nb);
     uint64_t t = 0;
     for (uint32_t b = 0; b < nb; ++b) {
       const uint32_t raw = (uint32_t)std::min<uint64_t>(B, P.n - (uint64_t)b * B);
@@ -643,6 +662,39 @@ int zgpu_serve_bench(zgpu_t* g, const zgpu_serve_cfg_t* cfg, const uint8_t* lz, 
   }
   if (rc == ZGPU_OK) rc = flush();
   rep->mismatches = bad;
+  // ---- the sample of responses, for the caller's own checks
+  if (cfg->sample_every && rc == ZGPU_OK) {
+    uint64_t bo = 0;
+    uint32_t lo = 0, ns = 0;
+    for (size_t i = 0; i < reqs.size(); i += cfg->sample_every) {
+      const SReq& r = reqs[i];
+      const Pool& P = pools[src_pool[i]];
+      const uint32_t nbk = srv.nblocks(&r);
+      const uint64_t len = r.op == 0 ? r.out_bytes : r.raw_len;
+      if (ns >= cfg->sample_cap || len > cfg->sample_bytes_cap - bo || (r.op == 0 && nbk > cfg->sample_blk_cap - lo))
+        break;
+      zgpu_serve_sample_t& sm = cfg->samples[ns++];
+      sm.op = (uint32_t)r.op;
+      sm.level = (uint32_t)r.level;
+      sm.pool = (uint32_t)src_pool[i];
+      sm.nblk = nbk;
+      sm.src_off = r.op == 0 ? src_off[i]
+                             : (uint64_t)(std::lower_bound(P.coff.begin(), P.coff.end(), src_off[i]) - P.coff.begin()) * B;
+      sm.raw_len = r.raw_len;
+      sm.bytes_off = bo;
+      sm.bytes_len = len;
+      memcpy(cfg->sample_bytes + bo, r.out, len);
+      bo += len;
+      sm.blk_len_off = lo;
+      if (r.op == 0) {
+        memcpy(cfg->sample_blk_len + lo, r.out_len, (size_t)nbk * 4);
+        lo += nbk;
+      }
+      sm.err = r.err;
+      sm.latency_ms = (r.t_done - r.t_arr) * 1e3;
+    }
+    rep->samples = ns;
+  }
   // ---- the same requests alone: each one its own batch, compared with the mix
   uint64_t alone_bad = 0;
   const size_t nalone = std::min<size_t>(reqs.size(), (size_t)cfg->alone_checks);

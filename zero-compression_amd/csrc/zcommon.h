@@ -37,6 +37,14 @@ struct Block {
   uint32_t level;
 };
 
+// a block's length as the encoder kernels take it: raw_len, or 0 when it exceeds the 2^blog slab that the block's
+// scratch is sized for (an entry no caller may pass: k_enc_pack then reports the block as BAD_BLOCK and nothing is
+// read or written past its slab)
+constexpr uint32_t BAD_BLOCK = 0xFFFFFFFEu;
+__host__ __device__ inline uint32_t enc_len(const Block& b, uint32_t blog) {
+  return b.raw_len <= (1u << blog) ? b.raw_len : 0u;
+}
+
 // per-block decoder error bits (err[b]); the host reports any as a damaged block
 enum : uint32_t {
   E_HDR = 1u << 0,     // payload shorter than its 20-byte header

@@ -22,6 +22,9 @@ ZGPU_ERR_ARG = -2
 ZGPU_ERR_DAMAGED = -3
 ZGPU_ERR_NOMEM = -4
 STORED = 0xFFFFFFFF
+BAD_BLOCK = 0xFFFFFFFE
+# per-block decoder error bits (err[b])
+E_HDR, E_COUNTS, E_STREAM, E_TABLE, E_LANE, E_TOKEN, E_GROUP, E_METHOD = (1 << k for k in range(8))
 
 # every symbol include/zero_hip.h declares (tests check the library exports them all)
 EXPORTS = (
@@ -31,7 +34,7 @@ EXPORTS = (
     "zgpu_event_create", "zgpu_event_destroy", "zgpu_event_record", "zgpu_event_elapsed_ms",
     "zgpu_encode_blocks", "zgpu_decode_blocks", "zgpu_crc32_blocks",
     "zgpu_encode_scratch_bytes", "zgpu_decode_scratch_bytes",
-    "zgpu_compress", "zgpu_decompress", "zgpu_crc32", "zgpu_crc32_ranges",
+    "zgpu_compress", "zgpu_decompress", "zgpu_encode_stream", "zgpu_crc32", "zgpu_crc32_ranges",
     "zgpu_set_profile", "zgpu_stage_times",
     "zgpu_plan_create", "zgpu_plan_destroy", "zgpu_plan_slab_bytes", "zgpu_plan_slab_offsets", "zgpu_plan_classes",
     "zgpu_plan_encode", "zgpu_plan_set_stored", "zgpu_plan_decode", "zgpu_plan_tune", "zgpu_plan_use_graphs",
@@ -109,6 +112,7 @@ def load_library(path=LIB_PATH):
         "zgpu_decode_scratch_bytes": (ctypes.c_size_t, [u32, u32]),
         "zgpu_compress": (i32, [vp, vp, u64, u32, i32, vp, vp, vp]),
         "zgpu_decompress": (i32, [vp, vp, vp, vp, vp, u32, u32, vp, vp, vp]),
+        "zgpu_encode_stream": (i32, [vp, vp, u32, vp, ctypes.POINTER(u32)]),
         "zgpu_crc32": (i32, [vp, vp, u64, u32, vp]),
         "zgpu_crc32_ranges": (i32, [vp, i32, u32, vp, vp, vp]),
         "zgpu_set_profile": (i32, [vp, i32]),
@@ -127,11 +131,22 @@ def _ptr(a):
     return ctypes.c_void_p(a.ctypes.data)
 
 
+class ServeSample(ctypes.Structure):
+    _fields_ = [("op", ctypes.c_uint32), ("level", ctypes.c_uint32), ("pool", ctypes.c_uint32),
+                ("nblk", ctypes.c_uint32), ("src_off", ctypes.c_uint64), ("raw_len", ctypes.c_uint64),
+                ("bytes_off", ctypes.c_uint64), ("bytes_len", ctypes.c_uint64), ("blk_len_off", ctypes.c_uint32),
+                ("err", ctypes.c_uint32), ("latency_ms", ctypes.c_double)]
+
+
 class ServeCfg(ctypes.Structure):
     _fields_ = [("offered_GBps", ctypes.c_double), ("seconds", ctypes.c_double), ("compress_share", ctypes.c_double),
                 ("random_share", ctypes.c_double), ("size_median", ctypes.c_double), ("size_sigma", ctypes.c_double),
                 ("size_max", ctypes.c_uint64), ("hbm_budget", ctypes.c_uint64), ("block_log", ctypes.c_uint32),
-                ("slots", ctypes.c_int), ("seed", ctypes.c_uint64), ("alone_checks", ctypes.c_uint32)]
+                ("slots", ctypes.c_int), ("seed", ctypes.c_uint64), ("alone_checks", ctypes.c_uint32),
+                ("dec_comp", vp * 2), ("dec_comp_len", vp * 2), ("dec_method", vp * 2),
+                ("sample_every", ctypes.c_uint32), ("sample_cap", ctypes.c_uint32),
+                ("samples", ctypes.POINTER(ServeSample)), ("sample_bytes", vp), ("sample_bytes_cap", ctypes.c_uint64),
+                ("sample_blk_len", vp), ("sample_blk_cap", ctypes.c_uint32)]
 
 
 class ServeReport(ctypes.Structure):
@@ -139,7 +154,7 @@ class ServeReport(ctypes.Structure):
                [(n, ctypes.c_double) for n in ("p50_ms", "p99_ms", "max_ms", "mean_ms", "wall_s", "sustained_GBps")] + \
                [("batches", ctypes.c_uint64), ("mean_batch_requests", ctypes.c_double)] + \
                [(n, ctypes.c_uint64) for n in ("hbm_bytes", "slot_input_bytes", "mismatches", "alone_checked",
-                                               "alone_mismatches")]
+                                               "alone_mismatches", "samples")]
 
 
 class DeviceBuffer:
@@ -247,6 +262,15 @@ class Gpu:
         self._check(rc)
         return out[: int(rlen.sum(dtype=np.uint64))].tobytes()
 
+    def encode_stream(self, sym):
+        """one stream section (kind, len, body) of the symbols, by the GPU's rANS encoder (zb::encode_stream)"""
+        a = np.frombuffer(bytes(sym), dtype=np.uint8) if not isinstance(sym, np.ndarray) else sym
+        out = np.empty(a.size + 512, dtype=np.uint8)
+        n = u32()
+        self._check(self.lib.zgpu_encode_stream(self.h, _ptr(a) if a.size else None, a.size, _ptr(out),
+                                                ctypes.byref(n)))
+        return out[: n.value].tobytes()
+
     def crc32_blocks_host(self, data, block_log):
         a = np.frombuffer(bytes(data), dtype=np.uint8) if not isinstance(data, np.ndarray) else data
         B = 1 << block_log
@@ -283,16 +307,43 @@ class Gpu:
 
     def serve_bench(self, lz_pool, random_pool, offered_GBps=20.0, seconds=1.0, compress_share=0.7,
                     random_share=0.1, size_median=256 * 1024, size_sigma=1.5, size_max=256 << 20,
-                    hbm_budget=64 << 30, block_log=18, slots=3, seed=5, alone_checks=64):
-        """The serving loop under an open-loop synthetic load (zgpu_serve_bench); returns its report as a dict."""
+                    hbm_budget=64 << 30, block_log=18, slots=3, seed=5, alone_checks=64, dec_pools=None,
+                    sample_every=0, sample_cap=256, sample_bytes=256 << 20):
+        """The serving loop under an open-loop synthetic load (zgpu_serve_bench); returns its report as a dict.
+        dec_pools: [(comp blob, comp_len[], method[])] for the LZ and the random pool, every 2^block_log block as the
+        caller compressed it (the decompress requests' inputs). sample_every > 0: the report's "sample" holds
+        (fields dict, response bytes, per-block payload sizes) of every sample_every-th request."""
         cfg = ServeCfg(offered_GBps, seconds, compress_share, random_share, size_median, size_sigma, size_max,
                        hbm_budget, block_log, slots, seed, alone_checks)
+        keep = []
+        if dec_pools is not None:
+            for i, (blob, clen, meth) in enumerate(dec_pools):
+                arrs = [np.ascontiguousarray(blob, dtype=np.uint8), np.ascontiguousarray(clen, dtype=np.uint32),
+                        np.ascontiguousarray(meth, dtype=np.uint8)]
+                keep += arrs
+                cfg.dec_comp[i], cfg.dec_comp_len[i], cfg.dec_method[i] = (a.ctypes.data for a in arrs)
+        if sample_every:
+            samples = (ServeSample * sample_cap)()
+            sbytes = np.empty(sample_bytes, dtype=np.uint8)
+            slens = np.empty(sample_bytes // 1024 + 1024, dtype=np.uint32)
+            cfg.sample_every, cfg.sample_cap, cfg.samples = sample_every, sample_cap, samples
+            cfg.sample_bytes, cfg.sample_bytes_cap = sbytes.ctypes.data, sbytes.size
+            cfg.sample_blk_len, cfg.sample_blk_cap = slens.ctypes.data, slens.size
         rep = ServeReport()
         lz = np.ascontiguousarray(lz_pool, dtype=np.uint8)
         rn = np.ascontiguousarray(random_pool, dtype=np.uint8)
         self._check(self.lib.zgpu_serve_bench(self.h, ctypes.byref(cfg), _ptr(lz), lz.size, _ptr(rn), rn.size,
                                               ctypes.byref(rep)))
-        return {n: getattr(rep, n) for n, _ in ServeReport._fields_}
+        out = {n: getattr(rep, n) for n, _ in ServeReport._fields_}
+        if sample_every:
+            out["sample"] = []
+            for k in range(rep.samples):
+                sm = samples[k]
+                f = {n: getattr(sm, n) for n, _ in ServeSample._fields_}
+                body = sbytes[f["bytes_off"]: f["bytes_off"] + f["bytes_len"]].tobytes()
+                lens = slens[f["blk_len_off"]: f["blk_len_off"] + f["nblk"]].copy() if f["op"] == 0 else None
+                out["sample"].append((f, body, lens))
+        return out
 
     def copy(self, dst, src, nbytes, stream=None):
         """device-to-device copy on the codec's stream (a plain kernel when both addresses are 16-byte aligned)"""
