@@ -107,6 +107,16 @@ int zgpu_encode_blocks(zgpu_t* g, const uint8_t* d_in, const zgpu_block_t* d_blo
  * the kernels access memory outside the block's payload, its scratch and its raw_len bytes at out + dst). */
 int zgpu_decode_blocks(zgpu_t* g, const uint8_t* d_comp, const zgpu_block_t* d_blocks, uint32_t nb, uint32_t block_log,
                        uint8_t* d_out, uint32_t* d_err, zgpu_stream_t s);
+/* zgpu_decode_blocks without any allocation. No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). The decoder's
+ * scratch lives in a fixed array of ZGPU_NOALLOC_SCRATCH_BYTES that the library holds on every GPU it runs on until
+ * the process ends: 128 blocks of 2^18, 32 of 2^20 or 512 of 2^16 per call (zgpu_decode_scratch_bytes(nb,
+ * block_log) at most that); a call that needs more is refused with ZGPU_ERR_ARG. The call allocates nothing and
+ * records no event. Every call on a GPU shares that GPU's array, which keeps the last call's contents: calls that
+ * overlap on one GPU (other threads, other streams, other contexts) race, so a caller runs them in turn. */
+#define ZGPU_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int zgpu_decode_blocks_noalloc(zgpu_t* g, const uint8_t* d_comp, const zgpu_block_t* d_blocks, uint32_t nb,
+                               uint32_t block_log, uint8_t* d_out, uint32_t* d_err, zgpu_stream_t s);
 /* crc[b] = CRC-32 of block b's raw bytes, at data + dst (use_dst != 0) or data + src (src/archive.h:376,618) */
 int zgpu_crc32_blocks(zgpu_t* g, const uint8_t* d_data, const zgpu_block_t* d_blocks, uint32_t nb, int use_dst,
                       uint32_t* d_crc, zgpu_stream_t s);

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -30,8 +31,35 @@ struct Stage {
 
 }  // namespace
 
+// The decoder's scratch for zgpu_decode_blocks_noalloc: a fixed array in the code object, so that a call allocates
+// nothing. Each GPU the library runs on holds its own copy (ZGPU_NOALLOC_SCRATCH_BYTES) until the process ends. In
+// host code `scratch` names a host-side stand-in: its device address on each GPU comes from hipGetSymbolAddress
+// (noalloc_scratch_of), and only that address is ever handed to kernels.
+static __device__ __align__(256) uint8_t scratch[ZGPU_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+void* scratch_ptr[MAX_DEVICES] = {};  // device address of `scratch` on each GPU, once looked up
+// the address of `scratch` on device dev (current), looked up once per device; nullptr on failure
+hipError_t noalloc_scratch_of(int dev, void** out) {
+  *out = nullptr;
+  if (dev < 0 || dev >= MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  if (!scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    scratch_ptr[dev] = p;
+  }
+  *out = scratch_ptr[dev];
+  return hipSuccess;
+}
+}  // namespace
+
 struct zgpu {
   int dev = 0;
+  void* noalloc = nullptr;  // this device's `scratch` (its device address)
   hipStream_t stream = nullptr;
   std::string err;
   zgpu_props_t props{};
@@ -215,6 +243,7 @@ int zgpu_open(int ordinal, zgpu_t** out) {
   hipDeviceProp_t p;
   if (e == hipSuccess) e = hipGetDeviceProperties(&p, ordinal);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = noalloc_scratch_of(ordinal, &g->noalloc);
   if (e != hipSuccess) { delete g; return ZGPU_ERR_DEVICE; }
   snprintf(g->props.name, sizeof g->props.name, "%s", p.name);
   snprintf(g->props.arch, sizeof g->props.arch, "%s", p.gcnArchName);
@@ -476,6 +505,24 @@ int zgpu_decode_blocks(zgpu_t* g, const uint8_t* d_comp, const zgpu_block_t* d_b
   hipStream_t st = s ? (hipStream_t)s : g->stream;
   if (int rc = ensure(g, g->dec, dec_layout(nb, blog).total)) return rc;
   return decode_run(g, d_comp, (const Block*)d_blocks, nb, blog, d_out, d_err, (uint8_t*)g->dec.p, st, true);
+}
+
+// zgpu_decode_blocks with the decoder's scratch in the fixed array `scratch` (through its device address): no
+// allocation, no event, no library call that allocates. Calls that overlap on one GPU share that array and race.
+int zgpu_decode_blocks_noalloc(zgpu_t* g, const uint8_t* d_comp, const zgpu_block_t* d_blocks, uint32_t nb,
+                               uint32_t blog, uint8_t* d_out, uint32_t* d_err, zgpu_stream_t s) {
+  if (blog < 16 || blog > 20) return fail(g, ZGPU_ERR_ARG, "block_log 16..20");
+  if (nb == 0) return ZGPU_OK;
+  const size_t need = dec_layout(nb, blog).total;
+  if (need > (size_t)ZGPU_NOALLOC_SCRATCH_BYTES) {
+    char m[160];
+    snprintf(m, sizeof m, "decode_blocks_noalloc: %u blocks of 2^%u need %zu bytes of scratch, more than the %llu it has",
+             nb, blog, need, (unsigned long long)ZGPU_NOALLOC_SCRATCH_BYTES);
+    return fail(g, ZGPU_ERR_ARG, m);
+  }
+  if (!g->noalloc) return fail(g, ZGPU_ERR_DEVICE, "decode_blocks_noalloc: no scratch address on this device");
+  hipStream_t st = s ? (hipStream_t)s : g->stream;
+  return decode_run(g, d_comp, (const Block*)d_blocks, nb, blog, d_out, d_err, (uint8_t*)g->noalloc, st, false);
 }
 
 int zgpu_crc32_blocks(zgpu_t* g, const uint8_t* d_data, const zgpu_block_t* d_blocks, uint32_t nb, int use_dst,

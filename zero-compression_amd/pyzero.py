@@ -22,6 +22,7 @@ ZGPU_ERR_ARG = -2
 ZGPU_ERR_DAMAGED = -3
 ZGPU_ERR_NOMEM = -4
 STORED = 0xFFFFFFFF
+NOALLOC_SCRATCH_BYTES = 64 << 20
 BAD_BLOCK = 0xFFFFFFFE
 # per-block decoder error bits (err[b])
 E_HDR, E_COUNTS, E_STREAM, E_TABLE, E_LANE, E_TOKEN, E_GROUP, E_METHOD = (1 << k for k in range(8))
@@ -32,7 +33,7 @@ EXPORTS = (
     "zgpu_alloc", "zgpu_free", "zgpu_alloc_host", "zgpu_free_host", "zgpu_upload", "zgpu_download", "zgpu_fill", "zgpu_copy",
     "zgpu_stream_create", "zgpu_stream_destroy", "zgpu_stream_sync",
     "zgpu_event_create", "zgpu_event_destroy", "zgpu_event_record", "zgpu_event_elapsed_ms",
-    "zgpu_encode_blocks", "zgpu_decode_blocks", "zgpu_crc32_blocks",
+    "zgpu_encode_blocks", "zgpu_decode_blocks", "zgpu_decode_blocks_noalloc", "zgpu_crc32_blocks",
     "zgpu_encode_scratch_bytes", "zgpu_decode_scratch_bytes",
     "zgpu_compress", "zgpu_decompress", "zgpu_encode_stream", "zgpu_crc32", "zgpu_crc32_ranges",
     "zgpu_set_profile", "zgpu_stage_times",
@@ -107,6 +108,7 @@ def load_library(path=LIB_PATH):
         "zgpu_event_elapsed_ms": (i32, [vp, vp, vp, ctypes.POINTER(ctypes.c_float)]),
         "zgpu_encode_blocks": (i32, [vp, vp, vp, u32, u32, i32, vp, vp, vp]),
         "zgpu_decode_blocks": (i32, [vp, vp, vp, u32, u32, vp, vp, vp]),
+        "zgpu_decode_blocks_noalloc": (i32, [vp, vp, vp, u32, u32, vp, vp, vp]),
         "zgpu_crc32_blocks": (i32, [vp, vp, vp, u32, i32, vp, vp]),
         "zgpu_encode_scratch_bytes": (ctypes.c_size_t, [u32, u32]),
         "zgpu_decode_scratch_bytes": (ctypes.c_size_t, [u32, u32]),
@@ -298,6 +300,11 @@ class Gpu:
 
     def decode_blocks(self, d_comp, d_blocks, nb, block_log, d_out, d_err, stream=None):
         self._check(self.lib.zgpu_decode_blocks(self.h, d_comp, d_blocks, nb, block_log, d_out, d_err, stream))
+
+    def decode_blocks_noalloc(self, d_comp, d_blocks, nb, block_log, d_out, d_err, stream=None):
+        """decode_blocks with the library's fixed scratch array: allocates nothing (ZGPU_NOALLOC_SCRATCH_BYTES)"""
+        self._check(self.lib.zgpu_decode_blocks_noalloc(self.h, d_comp, d_blocks, nb, block_log, d_out, d_err,
+                                                        stream))
 
     def crc32_blocks(self, d_data, d_blocks, nb, use_dst, d_crc, stream=None):
         self._check(self.lib.zgpu_crc32_blocks(self.h, d_data, d_blocks, nb, int(use_dst), d_crc, stream))
