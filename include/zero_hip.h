@@ -124,13 +124,16 @@ int zgpu_crc32_blocks(zgpu_t* g, const uint8_t* d_data, const zgpu_block_t* d_bl
 size_t zgpu_encode_scratch_bytes(uint32_t nb, uint32_t block_log);
 size_t zgpu_decode_scratch_bytes(uint32_t nb, uint32_t block_log);
 
-/* ---- the codec on host buffers: zero::Codec::compress / decompress (src/archive.h:56-69) as GpuCodec does them
- * (upload, kernels, download; synchronous). */
-/* n bytes = whole blocks, the last may be shorter. out: (nb << block_log) bytes, block b's payload at b << block_log;
- * out_len[b] = payload size, or ZGPU_STORED. crc (may be NULL): the CRC-32 of each block's raw bytes, which the
- * .zero container stores per block (src/archive.h:376). */
+/* ---- the codec on host buffers: zero::Codec::compress / decompress (src/archive.h:56-69) as GpuCodec does them.
+ * Synchronous. A batch moves in chunks over two streams, so the PCIe copies of one chunk overlap the kernels of the
+ * other (the reference uploads, runs and downloads in turn, src/gpu_codec.h:153-229,253-315); host buffers
+ * allocated with zgpu_alloc_host (pinned) move fastest and overlap best. */
+/* n bytes = whole blocks, the last may be shorter. out: the payloads of the compressed blocks back to back, in block
+ * order (a stored block takes no room; n bytes always suffice); out_len[b] = block b's payload size, or
+ * ZGPU_STORED. crc (may be NULL): the CRC-32 of each block's raw bytes, which the .zero container stores per block
+ * (src/archive.h:376). */
 int zgpu_compress(zgpu_t* g, const uint8_t* in, uint64_t n, uint32_t block_log, int level, uint8_t* out,
-                  uint32_t* out_len, uint32_t* crc);
+                  uint64_t out_cap, uint32_t* out_len, uint32_t* crc);
 /* blocks back to back in comp (comp_len[b] bytes each), their raw bytes back to back into out. err[b] as above;
  * returns ZGPU_ERR_DAMAGED when any block is damaged. crc (may be NULL): the CRC-32 of each decoded block, for the
  * container's check (src/archive.h:618). */

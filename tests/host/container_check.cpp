@@ -3,6 +3,7 @@
 // device copy. Batches hold 3 blocks, so files and their pieces cross batch boundaries. Test infrastructure.
 //   container_check c <out.zero> <block_log> <inputs...>   |   container_check t <archive>   |   x <archive> <dir>
 #include <cstdio>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -33,6 +34,7 @@ struct HostCodec : zero::Codec {
   size_t batch_bytes(uint32_t block_log) const override { return (size_t)3 << block_log; }
   bool range_crcs(bool input, const std::vector<uint64_t>& off, const std::vector<uint64_t>& len,
                   std::vector<uint32_t>& crc) override {
+    if (host_sums) return false;  // (the container then checksums the files' pieces itself)
     const std::vector<uint8_t>& d = input ? last_in : last_out;
     crc.resize(off.size());
     for (size_t i = 0; i < off.size(); ++i) {
@@ -43,6 +45,7 @@ struct HostCodec : zero::Codec {
     return true;
   }
   size_t calls = 0;
+  bool host_sums = getenv("CHECK_HOST_SUMS") != nullptr;
 };
 
 int main(int argc, char** argv) {

@@ -384,3 +384,25 @@ def test_block_table_checks(gpu):
             buf.free()
     for buf in (din, dtab, dout, dlen):
         buf.free()
+
+
+def test_host_buffer_forms_in_chunks(gpu):
+    """zgpu_compress / zgpu_decompress on 64 MiB at 2^16 (1024 blocks: four chunks over the two streams): payloads
+    packed back to back equal the oracle's model, block CRCs equal zlib's, the round trip is exact, and the range CRCs
+    cover the whole batch on the device"""
+    bl, B = 16, 1 << 16
+    d = pyzero.gen_lz(9, (1024 << 16) - 1234)
+    d[5 * B: 9 * B] = pyzero.gen_random(10, 4 * B)  # some stored blocks among the compressed ones
+    got = gpu.compress(d, bl, 5)
+    assert len(got) == 1024
+    assert [m for m, _ in got[5:9]] == [0, 0, 0, 0]
+    rng = random.Random(4)
+    for k in [0, 1, 255, 256, 511, 512, 767, 768, 1023] + rng.sample(range(1024), 12):
+        blk = d[k * B:(k + 1) * B].tobytes()
+        want, st = O.gpu_compress_block(blk, bl, 5)
+        assert got[k] == ((0, b"") if st else (1, want)), k
+    blocks = [(m, min(B, d.size - k * B), p if m else d[k * B:(k + 1) * B].tobytes()) for k, (m, p) in enumerate(got)]
+    assert gpu.decompress(blocks, bl) == d.tobytes()
+    assert gpu.last_crc.tolist() == [zlib.crc32(d[k * B:(k + 1) * B].tobytes()) for k in range(1024)]
+    spans = [(0, d.size), (B - 7, 3 * B + 11), (d.size - 5, 5), (600 * B + 3, 100 * B)]
+    assert gpu.crc32_ranges(1, spans) == [zlib.crc32(d[o: o + n].tobytes()) for o, n in spans]

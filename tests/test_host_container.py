@@ -37,18 +37,24 @@ def tree(root):
     return files
 
 
-def test_file_crcs_from_ranges_round_trip(tool, tmp_path):
+@pytest.mark.parametrize("host_sums", [False, True])
+def test_file_crcs_from_ranges_round_trip(tool, tmp_path, host_sums):
+    """files' CRCs from the codec's range CRCs, or (a codec without them) from the host; batches of 3 blocks, so
+    files and their pieces cross batch boundaries and the pipeline's slots are all reused"""
     files = tree(str(tmp_path))
+    env = {k: v for k, v in os.environ.items() if k != "CHECK_HOST_SUMS"}
+    if host_sums:
+        env["CHECK_HOST_SUMS"] = "1"
     arc = str(tmp_path / "a.zero")
-    r = subprocess.run([tool, "c", arc, "16", "t"], cwd=str(tmp_path), capture_output=True, text=True)
+    r = subprocess.run([tool, "c", arc, "16", "t"], cwd=str(tmp_path), capture_output=True, text=True, env=env)
     assert r.returncode == 0, r.stderr
-    assert "range_crc_calls=" in r.stdout and not r.stdout.strip().endswith("=0")
+    assert "range_crc_calls=" in r.stdout and r.stdout.strip().endswith("=0") == host_sums
     _, entries = read_archive(arc)  # (checks every block's and every file's CRC with zlib)
     assert {n: d for n, d in entries if d is not None} == files
-    r = subprocess.run([tool, "t", arc], capture_output=True, text=True)
+    r = subprocess.run([tool, "t", arc], capture_output=True, text=True, env=env)
     assert r.returncode == 0, r.stderr
     out = str(tmp_path / "out")
-    r = subprocess.run([tool, "x", arc, out], capture_output=True, text=True)
+    r = subprocess.run([tool, "x", arc, out], capture_output=True, text=True, env=env)
     assert r.returncode == 0, r.stderr
     for name, d in files.items():
         assert open(os.path.join(out, name), "rb").read() == d

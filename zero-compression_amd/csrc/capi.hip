@@ -57,16 +57,26 @@ hipError_t noalloc_scratch_of(int dev, void** out) {
 }
 }  // namespace
 
+// one of the two streams of the host-buffer forms, with what a chunk of a batch needs on it
+struct Lane {
+  hipStream_t st = nullptr;
+  hipEvent_t ev = nullptr;      // compress: the chunk's lengths and payload total are on the host
+  uint64_t* h_total = nullptr;  // (pinned) the chunk's payload bytes
+  DBuf work, slab, pay, meta;   // scratch; compress: slabs, packed payloads; block table, lengths, CRCs, offsets
+};
+
 struct zgpu {
   int dev = 0;
   void* noalloc = nullptr;  // this device's `scratch` (its device address)
+  Lane lanes[2];
+  DBuf hin, hout;           // host-buffer forms: the whole batch's input (compress, decompress) and output
   hipStream_t stream = nullptr;
   std::string err;
   zgpu_props_t props{};
   uint32_t* xpow = nullptr;  // x^(8 m 256^j) mod P, 3 x 256 (CRC combine)
-  DBuf enc, dec, aux;  // encoder scratch, decoder scratch, staging of the host-buffer forms
+  DBuf enc, dec, aux;  // encoder and decoder scratch of the device-resident forms; staging of the small host forms
   DBuf rng;            // byte-range CRC tables
-  // the last zgpu_compress input [0] and zgpu_decompress output [1] still in aux (length 0: none)
+  // the last zgpu_compress input [0] (in hin) and zgpu_decompress output [1] (in hout); length 0: none
   const uint8_t* last_data[2] = {nullptr, nullptr};
   uint64_t last_len[2] = {0, 0};
   bool profile = false;
@@ -281,7 +291,15 @@ void zgpu_close(zgpu_t* g) {
   if (!g) return;
   (void)hipSetDevice(g->dev);
   (void)hipStreamSynchronize(g->stream);
-  for (DBuf* b : {&g->enc, &g->dec, &g->aux, &g->rng})
+  for (Lane& l : g->lanes) {
+    if (l.st) (void)hipStreamSynchronize(l.st);
+    for (DBuf* b : {&l.work, &l.slab, &l.pay, &l.meta})
+      if (b->p) (void)hipFree(b->p);
+    if (l.h_total) (void)hipHostFree(l.h_total);
+    if (l.ev) (void)hipEventDestroy(l.ev);
+    if (l.st) (void)hipStreamDestroy(l.st);
+  }
+  for (DBuf* b : {&g->enc, &g->dec, &g->aux, &g->rng, &g->hin, &g->hout})
     if (b->p) (void)hipFree(b->p);
   for (hipEvent_t e : g->pool) (void)hipEventDestroy(e);
   if (g->xpow) (void)hipFree(g->xpow);
@@ -535,49 +553,140 @@ int zgpu_crc32_blocks(zgpu_t* g, const uint8_t* d_data, const zgpu_block_t* d_bl
 }
 
 // ---------------------------------------------------------------- host-buffer forms
-int zgpu_compress(zgpu_t* g, const uint8_t* in, uint64_t n, uint32_t blog, int level, uint8_t* out,
+// A batch moves in chunks of blocks over the two lanes (streams), chunk c on lane c % 2: while one lane's kernels
+// run, the other lane's copies move over PCIe, so upload, kernels and download overlap instead of taking turns. The
+// whole batch's input (compress) or output (decompress) stays on the device, for zgpu_crc32_ranges.
+namespace {
+// blocks per chunk of a batch of nb blocks of 2^blog: few chunks, so that each launch still fills the GPU (a
+// per-block kernel runs one wavefront or workgroup per block), but at least two from 512 blocks on, so that copies
+// have kernels to hide behind; at most 1 GiB of input per chunk, which bounds the lanes' scratch
+uint32_t chunk_blocks(uint32_t nb, uint32_t blog) {
+  const uint32_t chunks = std::max<uint32_t>(1, std::min<uint32_t>(4, nb / 256));
+  return std::min<uint32_t>((nb + chunks - 1) / chunks, (1u << 30) >> blog);
+}
+}  // namespace
+
+// the payload offsets of a chunk's blocks: goff[b] = the sizes of its compressed blocks before b (stored and bad
+// blocks take no room), *total = all of them; one workgroup, each thread a run of ceil(nb / 1024) blocks
+__global__ __launch_bounds__(1024) void k_payload_offsets(const uint32_t* __restrict__ len, uint32_t nb,
+                                                          uint64_t* __restrict__ goff, uint64_t* __restrict__ total) {
+  __shared__ uint64_t part[1024];
+  const uint32_t t = threadIdx.x, per = (nb + 1023) / 1024, b0 = t * per;
+  uint64_t s = 0;
+  for (uint32_t b = b0; b < min(nb, b0 + per); ++b) s += len[b] >= BAD_BLOCK ? 0u : len[b];
+  part[t] = s;
+  __syncthreads();
+  for (uint32_t d = 1; d < 1024; d <<= 1) {  // inclusive scan of the 1024 partial sums
+    const uint64_t v = t >= d ? part[t - d] : 0u;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  uint64_t o = part[t] - s;
+  for (uint32_t b = b0; b < min(nb, b0 + per); ++b) {
+    goff[b] = o;
+    o += len[b] >= BAD_BLOCK ? 0u : len[b];
+  }
+  if (t == 1023) *total = part[1023];
+}
+
+// block b's payload (len[b] bytes at slab + (b << blog); nothing when stored) to pay + goff[b]
+__global__ __launch_bounds__(256) void k_pack_payloads(const uint8_t* __restrict__ slab, uint32_t blog,
+                                                       const uint32_t* __restrict__ len, const uint64_t* __restrict__ goff,
+                                                       uint8_t* __restrict__ pay) {
+  const uint32_t b = blockIdx.x, n = len[b];
+  if (n >= BAD_BLOCK) return;
+  const uint8_t* s = slab + ((size_t)b << blog);
+  uint8_t* d = pay + goff[b];
+  const uint32_t full = n & ~15u;
+  for (uint32_t i = threadIdx.x * 16; i < full; i += 256 * 16) {
+    uint4 v;
+    __builtin_memcpy(&v, s + i, 16);
+    __builtin_memcpy(d + i, &v, 16);
+  }
+  if (full + threadIdx.x < n) d[full + threadIdx.x] = s[full + threadIdx.x];
+}
+
+static int lanes_ready(zgpu_t* g) {
+  for (Lane& l : g->lanes) {
+    if (!l.st) HIPCHK(g, hipStreamCreateWithFlags(&l.st, hipStreamNonBlocking));
+    if (!l.ev) HIPCHK(g, hipEventCreateWithFlags(&l.ev, hipEventDisableTiming));
+    if (!l.h_total) HIPCHK(g, hipHostMalloc((void**)&l.h_total, 64, hipHostMallocDefault));
+  }
+  return ZGPU_OK;
+}
+
+int zgpu_compress(zgpu_t* g, const uint8_t* in, uint64_t n, uint32_t blog, int level, uint8_t* out, uint64_t out_cap,
                   uint32_t* out_len, uint32_t* crc) {
   if (blog < 16 || blog > 20 || level < 1 || level > 9) return fail(g, ZGPU_ERR_ARG, "block_log 16..20, level 1..9");
   const uint64_t B = 1ull << blog;
   const uint32_t nb = (uint32_t)((n + B - 1) / B);
+  g->last_len[0] = g->last_len[1] = 0;
   if (nb == 0) return ZGPU_OK;
+  if (int rc = lanes_ready(g)) return rc;
+  const uint32_t per = chunk_blocks(nb, blog), nchunks = (nb + per - 1) / per;
+  const EncShape S = default_shape(blog);
+  // device: the whole input; per lane: scratch, slabs, packed payloads, the chunk's table, lengths, CRCs, offsets
+  if (int rc = ensure(g, g->hin, al(n) + 64)) return rc;
+  const size_t meta = al((size_t)per * 32) + 2 * al((size_t)per * 4) + al((size_t)per * 8) + al(8);
+  for (Lane& l : g->lanes) {
+    if (int rc = ensure(g, l.work, enc_layout(per, blog, S.hbits).total)) return rc;
+    if (int rc = ensure(g, l.slab, (size_t)per << blog)) return rc;
+    if (int rc = ensure(g, l.pay, (size_t)per << blog)) return rc;
+    if (int rc = ensure(g, l.meta, meta)) return rc;
+  }
+  uint8_t* d_in = (uint8_t*)g->hin.p;
   std::vector<zgpu_block_t> tab(nb);
   for (uint32_t b = 0; b < nb; ++b) {
-    tab[b].src = (uint64_t)b * B;
-    tab[b].dst = (uint64_t)b * B;
+    tab[b] = zgpu_block_t{};
+    tab[b].src = (uint64_t)(b % per) * B;  // (within its chunk's part of the input)
     tab[b].raw_len = (uint32_t)std::min<uint64_t>(B, n - (uint64_t)b * B);
-    tab[b].comp_len = 0;
-    tab[b].method = 0;
-    tab[b].level = 0;
   }
-  const size_t outb = (size_t)nb * B;
-  g->last_len[0] = g->last_len[1] = 0;
-  if (int rc = ensure(g, g->aux, al(n) + al(outb) + 2 * al((size_t)nb * 4) + al((size_t)nb * 32))) return rc;
-  uint8_t* d_in = (uint8_t*)g->aux.p;
+  auto front = [&](uint32_t c) -> int {  // chunk c: upload, kernels, its lengths and CRCs back
+    Lane& l = g->lanes[c & 1];
+    const uint32_t b0 = c * per, k = std::min(per, nb - b0);
+    const uint64_t o = (uint64_t)b0 * B, bytes = std::min<uint64_t>((uint64_t)k * B, n - o);
+    uint8_t* m = (uint8_t*)l.meta.p;
+    zgpu_block_t* d_tab = (zgpu_block_t*)m;
+    uint32_t* d_len = (uint32_t*)(m + al((size_t)per * 32));
+    uint32_t* d_crc = (uint32_t*)((uint8_t*)d_len + al((size_t)per * 4));
+    uint64_t* d_goff = (uint64_t*)((uint8_t*)d_crc + al((size_t)per * 4));
+    uint64_t* d_tot = (uint64_t*)((uint8_t*)d_goff + al((size_t)per * 8));
+    HIPCHK(g, hipMemcpyAsync(d_in + o, in + o, bytes, hipMemcpyHostToDevice, l.st));
+    HIPCHK(g, hipMemcpyAsync(d_tab, tab.data() + b0, (size_t)k * 32, hipMemcpyHostToDevice, l.st));
+    if (int rc = encode_run(g, d_in + o, (const Block*)d_tab, k, S, level, (uint8_t*)l.slab.p, d_len,
+                            (uint8_t*)l.work.p, l.st, c == 0))
+      return rc;
+    k_payload_offsets<<<1, 1024, 0, l.st>>>(d_len, k, d_goff, d_tot);
+    k_pack_payloads<<<k, 256, 0, l.st>>>((const uint8_t*)l.slab.p, blog, d_len, d_goff, (uint8_t*)l.pay.p);
+    HIPCHK(g, hipGetLastError());
+    if (crc) {
+      k_crc32<<<(k + 3) / 4, 256, 0, l.st>>>((const Block*)d_tab, k, d_in + o, 0, g->xpow, d_crc);
+      HIPCHK(g, hipGetLastError());
+      HIPCHK(g, hipMemcpyAsync(crc + b0, d_crc, (size_t)k * 4, hipMemcpyDeviceToHost, l.st));
+    }
+    HIPCHK(g, hipMemcpyAsync(out_len + b0, d_len, (size_t)k * 4, hipMemcpyDeviceToHost, l.st));
+    HIPCHK(g, hipMemcpyAsync(l.h_total, d_tot, 8, hipMemcpyDeviceToHost, l.st));
+    HIPCHK(g, hipEventRecord(l.ev, l.st));
+    return ZGPU_OK;
+  };
+  if (int rc = front(0)) return rc;
+  if (nchunks > 1)
+    if (int rc = front(1)) return rc;
+  uint64_t base = 0;
+  for (uint32_t c = 0; c < nchunks; ++c) {
+    Lane& l = g->lanes[c & 1];
+    HIPCHK(g, hipEventSynchronize(l.ev));  // (the other lane's chunk keeps the GPU busy meanwhile)
+    const uint64_t total = *l.h_total;
+    if (total > out_cap - base) return fail(g, ZGPU_ERR_ARG, "compress: the output buffer is too small");
+    if (total) HIPCHK(g, hipMemcpyAsync(out + base, l.pay.p, total, hipMemcpyDeviceToHost, l.st));
+    base += total;
+    if (c + 2 < nchunks)
+      if (int rc = front(c + 2)) return rc;  // (after this lane's download: the stream keeps them in order)
+  }
+  for (Lane& l : g->lanes) HIPCHK(g, hipStreamSynchronize(l.st));
   g->last_data[0] = d_in;
   g->last_len[0] = n;
-  uint8_t* d_out = d_in + al(n);
-  uint32_t* d_len = (uint32_t*)(d_out + al(outb));
-  uint32_t* d_crc = (uint32_t*)((uint8_t*)d_len + al((size_t)nb * 4));
-  zgpu_block_t* d_tab = (zgpu_block_t*)((uint8_t*)d_crc + al((size_t)nb * 4));
-  HIPCHK(g, hipMemcpyAsync(d_in, in, n, hipMemcpyHostToDevice, g->stream));
-  HIPCHK(g, hipMemcpyAsync(d_tab, tab.data(), (size_t)nb * 32, hipMemcpyHostToDevice, g->stream));
-  if (int rc = zgpu_encode_blocks(g, d_in, d_tab, nb, blog, level, d_out, d_len, nullptr)) return rc;
-  if (crc) {
-    if (int rc = zgpu_crc32_blocks(g, d_in, d_tab, nb, 0, d_crc, nullptr)) return rc;
-    HIPCHK(g, hipMemcpyAsync(crc, d_crc, (size_t)nb * 4, hipMemcpyDeviceToHost, g->stream));
-  }
-  HIPCHK(g, hipMemcpyAsync(out_len, d_len, (size_t)nb * 4, hipMemcpyDeviceToHost, g->stream));
-  HIPCHK(g, hipStreamSynchronize(g->stream));
-  // the payloads: one read of the slabs from the first to the last compressed block (src/gpu_codec.h:210-218)
-  uint32_t first = nb, last = 0;
-  for (uint32_t b = 0; b < nb; ++b)
-    if (out_len[b] != ZGPU_STORED) { if (b < first) first = b; last = b; }
-  if (first < nb) {
-    HIPCHK(g, hipMemcpyAsync(out + (size_t)first * B, d_out + (size_t)first * B, (size_t)(last - first + 1) * B,
-                             hipMemcpyDeviceToHost, g->stream));
-    HIPCHK(g, hipStreamSynchronize(g->stream));
-  }
   for (uint32_t b = 0; b < nb; ++b)
     if (out_len[b] != ZGPU_STORED && out_len[b] >= tab[b].raw_len)
       return fail(g, ZGPU_ERR_DEVICE, "GPU encoder produced an impossible block size");
@@ -587,45 +696,63 @@ int zgpu_compress(zgpu_t* g, const uint8_t* in, uint64_t n, uint32_t blog, int l
 int zgpu_decompress(zgpu_t* g, const uint8_t* comp, const uint32_t* comp_len, const uint8_t* method,
                     const uint32_t* raw_len, uint32_t nb, uint32_t blog, uint8_t* out, uint32_t* err, uint32_t* crc) {
   if (blog < 16 || blog > 20) return fail(g, ZGPU_ERR_ARG, "block_log 16..20");
+  g->last_len[0] = g->last_len[1] = 0;
   if (nb == 0) return ZGPU_OK;
   const uint64_t B = 1ull << blog;
   std::vector<zgpu_block_t> tab(nb);
-  uint64_t co = 0, ro = 0;
+  std::vector<uint64_t> co(nb + 1), ro(nb + 1);
+  co[0] = ro[0] = 0;
   for (uint32_t b = 0; b < nb; ++b) {
     if (raw_len[b] > B) return fail(g, ZGPU_ERR_DAMAGED, "block larger than the archive's block size");
-    tab[b].src = co;
-    tab[b].dst = ro;
-    tab[b].raw_len = raw_len[b];
-    tab[b].comp_len = comp_len[b];
-    tab[b].method = method[b];
-    tab[b].level = 0;
-    co += comp_len[b];
-    ro += raw_len[b];
+    co[b + 1] = co[b] + comp_len[b];
+    ro[b + 1] = ro[b] + raw_len[b];
   }
-  g->last_len[0] = g->last_len[1] = 0;
-  if (int rc = ensure(g, g->aux, al(co) + al(ro) + 2 * al((size_t)nb * 4) + al((size_t)nb * 32))) return rc;
-  uint8_t* d_comp = (uint8_t*)g->aux.p;
-  uint8_t* d_out = d_comp + al(co);
+  if (int rc = lanes_ready(g)) return rc;
+  const uint32_t per = chunk_blocks(nb, blog), nchunks = (nb + per - 1) / per;
+  if (int rc = ensure(g, g->hin, al(co[nb]) + 64)) return rc;
+  if (int rc = ensure(g, g->hout, al(ro[nb]) + 64)) return rc;
+  for (Lane& l : g->lanes) {
+    if (int rc = ensure(g, l.work, dec_layout(per, blog).total)) return rc;
+    if (int rc = ensure(g, l.meta, al((size_t)per * 32) + 2 * al((size_t)per * 4))) return rc;
+  }
+  uint8_t* d_comp = (uint8_t*)g->hin.p;
+  uint8_t* d_out = (uint8_t*)g->hout.p;
+  for (uint32_t c = 0; c < nchunks; ++c) {  // every size is known: the chunks are queued at once
+    Lane& l = g->lanes[c & 1];
+    const uint32_t b0 = c * per, k = std::min(per, nb - b0);
+    for (uint32_t b = b0; b < b0 + k; ++b) {
+      tab[b].src = co[b] - co[b0];
+      tab[b].dst = ro[b] - ro[b0];
+      tab[b].raw_len = raw_len[b];
+      tab[b].comp_len = comp_len[b];
+      tab[b].method = method[b];
+      tab[b].level = 0;
+    }
+    uint8_t* m = (uint8_t*)l.meta.p;
+    zgpu_block_t* d_tab = (zgpu_block_t*)m;
+    uint32_t* d_err = (uint32_t*)(m + al((size_t)per * 32));
+    uint32_t* d_crc = (uint32_t*)((uint8_t*)d_err + al((size_t)per * 4));
+    HIPCHK(g, hipMemcpyAsync(d_comp + co[b0], comp + co[b0], co[b0 + k] - co[b0], hipMemcpyHostToDevice, l.st));
+    HIPCHK(g, hipMemcpyAsync(d_tab, tab.data() + b0, (size_t)k * 32, hipMemcpyHostToDevice, l.st));
+    if (int rc = decode_run(g, d_comp + co[b0], (const Block*)d_tab, k, blog, d_out + ro[b0], d_err,
+                            (uint8_t*)l.work.p, l.st, c == 0))
+      return rc;
+    if (crc) {
+      k_crc32<<<(k + 3) / 4, 256, 0, l.st>>>((const Block*)d_tab, k, d_out + ro[b0], 1, g->xpow, d_crc);
+      HIPCHK(g, hipGetLastError());
+      HIPCHK(g, hipMemcpyAsync(crc + b0, d_crc, (size_t)k * 4, hipMemcpyDeviceToHost, l.st));
+    }
+    HIPCHK(g, hipMemcpyAsync(err + b0, d_err, (size_t)k * 4, hipMemcpyDeviceToHost, l.st));
+    HIPCHK(g, hipMemcpyAsync(out + ro[b0], d_out + ro[b0], ro[b0 + k] - ro[b0], hipMemcpyDeviceToHost, l.st));
+  }
+  for (Lane& l : g->lanes) HIPCHK(g, hipStreamSynchronize(l.st));
   g->last_data[1] = d_out;
-  g->last_len[1] = ro;
-  uint32_t* d_err = (uint32_t*)(d_out + al(ro));
-  uint32_t* d_crc = (uint32_t*)((uint8_t*)d_err + al((size_t)nb * 4));
-  zgpu_block_t* d_tab = (zgpu_block_t*)((uint8_t*)d_crc + al((size_t)nb * 4));
-  HIPCHK(g, hipMemcpyAsync(d_comp, comp, co, hipMemcpyHostToDevice, g->stream));
-  HIPCHK(g, hipMemcpyAsync(d_tab, tab.data(), (size_t)nb * 32, hipMemcpyHostToDevice, g->stream));
-  if (int rc = zgpu_decode_blocks(g, d_comp, d_tab, nb, blog, d_out, d_err, nullptr)) return rc;
-  if (crc) {
-    if (int rc = zgpu_crc32_blocks(g, d_out, d_tab, nb, 1, d_crc, nullptr)) return rc;
-    HIPCHK(g, hipMemcpyAsync(crc, d_crc, (size_t)nb * 4, hipMemcpyDeviceToHost, g->stream));
-  }
-  HIPCHK(g, hipMemcpyAsync(err, d_err, (size_t)nb * 4, hipMemcpyDeviceToHost, g->stream));
-  HIPCHK(g, hipMemcpyAsync(out, d_out, ro, hipMemcpyDeviceToHost, g->stream));
-  HIPCHK(g, hipStreamSynchronize(g->stream));
+  g->last_len[1] = ro[nb];
   for (uint32_t b = 0; b < nb; ++b)
     if (err[b]) {
-      char m[96];
-      snprintf(m, sizeof m, "block %u damaged (GPU decoder, code 0x%x)", b, err[b]);
-      return fail(g, ZGPU_ERR_DAMAGED, m);
+      char msg[96];
+      snprintf(msg, sizeof msg, "block %u damaged (GPU decoder, code 0x%x)", b, err[b]);
+      return fail(g, ZGPU_ERR_DAMAGED, msg);
     }
   return ZGPU_OK;
 }

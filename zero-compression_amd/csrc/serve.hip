@@ -479,15 +479,17 @@ int zgpu_serve_bench(zgpu_t* g, const zgpu_serve_cfg_t* cfg, const uint8_t* lz, 
       continue;
     }
     std::vector<uint32_t> len(nb);
-    uint8_t* slabs = (uint8_t*)malloc((size_t)nb * B);
-    if (!slabs) return fail(g, ZGPU_ERR_NOMEM, "serve: host memory");
+    uint8_t* pays = (uint8_t*)malloc((size_t)P.n + 1);
+    if (!pays) return fail(g, ZGPU_ERR_NOMEM, "serve: host memory");
+    uint64_t used = 0;
     for (uint32_t b0 = 0; b0 < nb && rc == ZGPU_OK; b0 += 4096) {
       const uint32_t k = std::min<uint32_t>(4096, nb - b0);
       const uint64_t o = (uint64_t)b0 * B, n = std::min<uint64_t>((uint64_t)k * B, P.n - o);
-      rc = zgpu_compress(g, P.raw + o, n, blog, 5, slabs + o, len.data() + b0, nullptr);
+      rc = zgpu_compress(g, P.raw + o, n, blog, 5, pays + used, P.n - used, len.data() + b0, nullptr);
+      for (uint32_t b = b0; b < b0 + k && rc == ZGPU_OK; ++b) used += len[b] == ZGPU_STORED ? 0 : len[b];
     }
-    if (rc) { free(slabs); return rc; }
-    uint64_t t = 0;
+    if (rc) { free(pays); return rc; }
+    uint64_t t = 0, q = 0;
     for (uint32_t b = 0; b < nb; ++b) {
       const uint32_t raw = (uint32_t)std::min<uint64_t>(B, P.n - (uint64_t)b * B);
       P.meth[b] = len[b] == ZGPU_STORED ? 0 : 1;
@@ -497,9 +499,11 @@ int zgpu_serve_bench(zgpu_t* g, const zgpu_serve_cfg_t* cfg, const uint8_t* lz, 
     }
     P.coff[nb] = t;
     HIPCHK(g, hipHostMalloc((void**)&P.comp, t ? t : 1, hipHostMallocDefault));
-    for (uint32_t b = 0; b < nb; ++b)
-      memcpy(P.comp + P.coff[b], P.meth[b] ? slabs + (uint64_t)b * B : P.raw + (uint64_t)b * B, P.clen[b]);
-    free(slabs);
+    for (uint32_t b = 0; b < nb; ++b) {  // (the payloads come back to back, in block order)
+      memcpy(P.comp + P.coff[b], P.meth[b] ? pays + q : P.raw + (uint64_t)b * B, P.clen[b]);
+      if (P.meth[b]) q += P.clen[b];
+    }
+    free(pays);
   }
   // ---- the requests: Poisson arrivals, lognormal sizes, 70 % compress, LZ : random = 9 : 1, levels 1 / 5 / 9
   std::mt19937_64 rng(cfg->seed);

@@ -1,44 +1,39 @@
-// The .zero container (format: src/archive.h:1-11; pipelines: src/archive.h:170-785), over any zero::Codec.
+// Compressing files and folders into a .zero archive, and listing, testing and extracting one, over any zero::Codec
+// (src/archive.h:334-785 is the reference's container; the byte layout, in format.h, is the same).
 //
-//   header   24 B: "ZERO", u8 version 2, u8 block_log (16..20), u16 0, u64 total raw bytes, u64 blocks
-//   block    u32 raw_len, u32 comp_len, u8 method (0 stored, 1 ZB2), 3 x u8 0, u32 CRC-32 of the raw bytes, payload
-//   table    u32 entries; each: u8 type (1 folder), 3 x u8 0, u32 name length, UTF-8 name ('/'), u64 size,
-//            i64 mtime (Unix seconds), u32 attributes (0), u32 CRC-32 of the file
-//   footer   24 B: u64 table offset, u64 table length, u32 CRC-32 of the table, "ZEND"
-//
-// The files' bytes, in entry order, form one solid stream cut into blocks; every block decompresses on its own.
-// Block checksums come from the codec when it computes them (the GPU does), else from the host.
+// Both directions run as a pipeline over three batch slots: while the codec works on batch k, one worker thread
+// reads batch k + 1 (from the files, or from the archive) and another checks and writes batch k - 1. A slot is
+// reused for batch k + 3 only after batch k's write is done, and the codec's own buffers (payload views, decoded
+// bytes) stay valid through the call after the one that filled them, which the write of batch k needs while batch
+// k + 1 is in the codec. Checksums come from the codec where it can compute them (block_crcs, range_crcs: on the
+// GPU, where the batch is), else from the host.
 #pragma once
-#include <sys/stat.h>
 #include <fcntl.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
-#include <algorithm>
 #include <atomic>
-#include <chrono>
-#include <cstdio>
-#include <filesystem>
+#include <condition_variable>
+#include <deque>
 #include <functional>
 #include <future>
-#include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
 
 #include "codec.h"
 #include "crc32.h"
+#include "format.h"
+#include "solid.h"
+#include "tree.h"
 
 namespace zero {
 
-namespace fs = std::filesystem;
-constexpr uint32_t MIN_BLOCK_LOG = 16, MAX_BLOCK_LOG = 20;
-
-inline double now() {
-  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
-}
-
+// what the user interface hears while an operation runs
 struct Report {
   std::function<void(const std::string&)> log;
-  std::function<void(uint64_t done, uint64_t total)> progress;
+  std::function<void(uint64_t done, uint64_t total)> progress;  // bytes of the solid stream done
   std::atomic<bool>* cancel = nullptr;
   void say(const std::string& s) const { if (log) log(s); }
   void at(uint64_t d, uint64_t t) const { if (progress) progress(d, t); }
@@ -48,613 +43,483 @@ struct Cancelled : std::runtime_error {
   Cancelled() : std::runtime_error("cancelled") {}
 };
 
-struct Entry {
-  bool dir = false;
-  std::string name;  // relative, '/' separated
-  uint64_t size = 0;
-  int64_t mtime = 0;
-  uint32_t attr = 0;
-  uint32_t crc = 0;
-  fs::path source;
-};
-
-inline int64_t mtime_of(const fs::path& p) {
-  struct stat st;
-  return ::stat(p.c_str(), &st) == 0 ? (int64_t)st.st_mtim.tv_sec : 0;
-}
-inline void set_mtime(const fs::path& p, int64_t secs) {
-  if (!secs) return;
-  struct timespec ts[2];
-  ts[0].tv_sec = ts[1].tv_sec = (time_t)secs;
-  ts[0].tv_nsec = ts[1].tv_nsec = 0;
-  ::utimensat(AT_FDCWD, p.c_str(), ts, 0);
-}
-
-// a file is one entry under its own name; a folder is itself and everything in it, sorted inside each folder;
-// symbolic links are not followed; a name already taken keeps its first entry (src/archive.h:168-240)
-inline std::vector<Entry> collect(const std::vector<std::string>& inputs, Report& rep) {
-  std::vector<Entry> es;
-  std::vector<std::string> seen;
-  auto add = [&](Entry e) {
-    if (std::find(seen.begin(), seen.end(), e.name) != seen.end()) {
-      rep.say("skipped (same name already in the archive): " + e.name);
-      return;
-    }
-    seen.push_back(e.name);
-    es.push_back(std::move(e));
-  };
-  std::function<void(const fs::path&, const std::string&)> walk = [&](const fs::path& dir, const std::string& rel) {
-    std::vector<fs::directory_entry> items;
-    std::error_code ec;
-    for (fs::directory_iterator it(dir, fs::directory_options::skip_permission_denied, ec), end; !ec && it != end;
-         it.increment(ec))
-      items.push_back(*it);
-    std::sort(items.begin(), items.end(), [](const fs::directory_entry& a, const fs::directory_entry& b) {
-      return a.path().filename().string() < b.path().filename().string();
-    });
-    for (const auto& it : items) {
-      if (rep.cancelled()) throw Cancelled();
-      const std::string name = rel + "/" + it.path().filename().string();
-      std::error_code e2;
-      if (it.is_symlink(e2)) continue;
-      if (it.is_directory(e2)) {
-        Entry d;
-        d.dir = true;
-        d.name = name;
-        d.mtime = mtime_of(it.path());
-        d.source = it.path();
-        add(std::move(d));
-        walk(it.path(), name);
-      } else if (it.is_regular_file(e2)) {
-        Entry f;
-        f.name = name;
-        f.size = it.file_size(e2);
-        if (e2) f.size = 0;
-        f.mtime = mtime_of(it.path());
-        f.source = it.path();
-        add(std::move(f));
-      }
-    }
-  };
-  for (const std::string& in : inputs) {
-    std::error_code ec;
-    fs::path p = fs::absolute(fs::path(in), ec);
-    while (!p.empty() && !p.has_filename()) p = p.parent_path();
-    if (!fs::exists(p, ec)) throw Error("not found: " + in);
-    const std::string top = p.filename().string();
-    Entry e;
-    e.name = top;
-    e.mtime = mtime_of(p);
-    e.source = p;
-    if (fs::is_directory(p, ec)) {
-      e.dir = true;
-      add(std::move(e));
-      walk(p, top);
-    } else if (fs::is_regular_file(p, ec)) {
-      e.size = fs::file_size(p, ec);
-      add(std::move(e));
-    } else {
-      rep.say("skipped (not a file or folder): " + in);
-    }
-  }
-  return es;
-}
-
-inline void put32(std::vector<uint8_t>& o, uint32_t v) { for (int i = 0; i < 4; ++i) o.push_back((uint8_t)(v >> (8 * i))); }
-inline void put64(std::vector<uint8_t>& o, uint64_t v) { put32(o, (uint32_t)v); put32(o, (uint32_t)(v >> 32)); }
-inline uint32_t get32(const uint8_t* p) { uint32_t v; memcpy(&v, p, 4); return v; }
-inline uint64_t get64(const uint8_t* p) { uint64_t v; memcpy(&v, p, 8); return v; }
-
 struct CompressStats {
   uint64_t raw = 0, compressed = 0, blocks = 0, stored_blocks = 0;
   size_t files = 0, folders = 0;
   double seconds = 0, codec_seconds = 0;
 };
-
-// Each file's CRC-32 from the CRCs of its pieces in the batches of the solid stream, which the codec computes where
-// the batch already is (on the GPU): crc(A||B) = combine(crc(A), crc(B), |B|). The reference checksums files on the
-// host as it reads and writes them (src/archive.h:279-287,320,707,774). finished(i, crc) is called for every file
-// once all its bytes have gone by, in entry order (empty files with CRC 0).
-struct FileCrcs {
-  const std::vector<Entry>& es;
-  size_t idx = 0;         // the entry the next byte of the stream belongs to
-  uint64_t in_file = 0;   // its bytes already seen
-  uint32_t crc = 0;       // their CRC
-  explicit FileCrcs(const std::vector<Entry>& e) : es(e) {}
-  template <class F> void settle(F&& finished) {
-    while (idx < es.size()) {
-      const Entry& e = es[idx];
-      if (e.dir) { ++idx; continue; }
-      if (in_file < e.size) return;
-      finished(idx, e.size ? crc : 0u);
-      ++idx;
-      in_file = 0;
-      crc = 0;
-    }
-  }
-  // the next n bytes of the stream, the last compress() input (input) or decompress() output of the codec
-  template <class F> void batch(Codec& codec, bool input, uint64_t n, F&& finished) {
-    std::vector<uint64_t> off, len;
-    uint64_t at = 0;
-    for (size_t i = idx, d = in_file; at < n;) {  // (the pieces, as the fold below will meet them)
-      if (i >= es.size()) throw Error("archive damaged: more data than the table lists");
-      const Entry& e = es[i];
-      if (e.dir || d >= e.size) { ++i; d = 0; continue; }
-      const uint64_t take = std::min<uint64_t>(n - at, e.size - d);
-      off.push_back(at);
-      len.push_back(take);
-      at += take;
-      d += take;
-    }
-    std::vector<uint32_t> pc;
-    if (!codec.range_crcs(input, off, len, pc)) throw Error("the codec lost its checksum support");
-    settle(finished);
-    for (size_t k = 0; k < off.size(); ++k) {
-      crc = crc::combine(crc, pc[k], len[k]);
-      in_file += len[k];
-      settle(finished);
-    }
-  }
-  template <class F> void close(F&& finished) { settle(finished); }
-};
-
-// the entries' bytes in order, in batches, with each file's CRC (src/archive.h:289-332) unless the codec computes
-// those (host_crc false)
-struct SolidReader {
-  std::vector<Entry>& es;
-  size_t idx = 0;
-  uint64_t pos = 0;
-  FILE* f = nullptr;
-  uint32_t crc = 0;
-  bool host_crc = true;
-  explicit SolidReader(std::vector<Entry>& e) : es(e) {}
-  ~SolidReader() { if (f) fclose(f); }
-  size_t read(uint8_t* buf, size_t n) {
-    size_t got = 0;
-    while (got < n && idx < es.size()) {
-      Entry& e = es[idx];
-      if (e.dir || pos >= e.size) {
-        if (!e.dir && host_crc) e.crc = crc;
-        if (f) { fclose(f); f = nullptr; }
-        crc = 0;
-        ++idx;
-        pos = 0;
-        continue;
-      }
-      if (!f && !(f = fopen(e.source.c_str(), "rb"))) throw Error("cannot read " + e.source.string());
-      const size_t want = (size_t)std::min<uint64_t>(n - got, e.size - pos);
-      const size_t r = fread(buf + got, 1, want, f);
-      if (r != want) memset(buf + got + r, 0, want - r);  // (a file that shrank: zeros keep the table right)
-      if (host_crc) crc = crc::update(crc, buf + got, want);
-      got += want;
-      pos += want;
-    }
-    return got;
-  }
-  void finish() {
-    if (host_crc && idx < es.size() && !es[idx].dir && pos >= es[idx].size) es[idx].crc = crc;
-    if (f) { fclose(f); f = nullptr; }
-    idx = es.size();
-  }
-};
-
-inline CompressStats compress_archive(const std::vector<std::string>& inputs, const std::string& out_path, Codec& codec,
-                                      uint32_t block_log, Report& rep) {
-  if (block_log < MIN_BLOCK_LOG || block_log > MAX_BLOCK_LOG) throw Error("block size must be 64 KiB .. 1 MiB (log2 16..20)");
-  const double t0 = now();
-  CompressStats st;
-  std::vector<Entry> es = collect(inputs, rep);
-  uint64_t total = 0;
-  for (const Entry& e : es) { total += e.size; if (e.dir) ++st.folders; else ++st.files; }
-  st.raw = total;
-  rep.say(std::to_string(st.files) + " files, " + std::to_string(st.folders) + " folders, " + std::to_string(total) + " bytes");
-  FILE* f = fopen(out_path.c_str(), "wb");
-  if (!f) throw Error("cannot create " + out_path);
-  struct Closer {  // (an archive left unfinished is removed)
-    FILE* f;
-    std::string p;
-    ~Closer() { if (f) { fclose(f); std::remove(p.c_str()); } }
-  } closer{f, out_path};
-  const size_t B = (size_t)1 << block_log;
-  std::vector<uint8_t> hdr;
-  hdr.insert(hdr.end(), {'Z', 'E', 'R', 'O', 2, (uint8_t)block_log, 0, 0});
-  put64(hdr, total);
-  put64(hdr, (total + B - 1) / B);
-  fwrite(hdr.data(), 1, hdr.size(), f);
-  size_t batch = std::max(B, codec.batch_bytes(block_log) / B * B);
-  std::vector<uint8_t> buf(batch), next_buf(batch);
-  SolidReader reader(es);
-  FileCrcs files(es);
-  std::vector<uint32_t> probe;
-  const bool codec_crc = codec.range_crcs(true, {}, {}, probe);  // (the files' CRCs on the GPU when it can)
-  reader.host_crc = !codec_crc;
-  auto set_crc = [&es](size_t i, uint32_t c) { es[i].crc = c; };
-  uint64_t done = 0, queued = 0;
-  std::vector<BlockOut> outs;
-  rep.at(0, total);
-  auto fill = [&reader, batch](std::vector<uint8_t>& b, uint64_t left) {
-    return reader.read(b.data(), (size_t)std::min<uint64_t>(batch, left));
-  };
-  size_t n = fill(buf, total);
-  queued = n;
-  std::vector<uint32_t> host_crc;
-  while (n) {
-    if (rep.cancelled()) throw Cancelled();
-    std::future<size_t> ahead;  // the next batch is read while the GPU works on this one
-    if (queued < total) ahead = std::async(std::launch::async, fill, std::ref(next_buf), total - queued);
-    const double c0 = now();
-    codec.compress(buf.data(), n, block_log, outs);
-    if (codec_crc) files.batch(codec, true, n, set_crc);
-    st.codec_seconds += now() - c0;
-    const uint32_t* crcs = codec.block_crcs();
-    if (!crcs) {
-      host_crc.resize(outs.size());
-      for (size_t b = 0; b < outs.size(); ++b) host_crc[b] = crc::of(buf.data() + b * B, std::min(B, n - b * B));
-      crcs = host_crc.data();
-    }
-    for (size_t b = 0; b < outs.size(); ++b) {
-      const size_t off = b * B;
-      const uint32_t len = (uint32_t)std::min(B, n - off);
-      const BlockOut& o = outs[b];
-      const bool stored = o.method == 0 || o.data.size() >= len;
-      std::vector<uint8_t> bh;
-      put32(bh, len);
-      put32(bh, stored ? len : (uint32_t)o.data.size());
-      bh.insert(bh.end(), {(uint8_t)(stored ? 0 : o.method), 0, 0, 0});
-      put32(bh, crcs[b]);
-      fwrite(bh.data(), 1, bh.size(), f);
-      if (stored) { fwrite(buf.data() + off, 1, len, f); ++st.stored_blocks; st.compressed += len; }
-      else { fwrite(o.data.data(), 1, o.data.size(), f); st.compressed += o.data.size(); }
-      ++st.blocks;
-    }
-    done += n;
-    rep.at(done, total);
-    n = 0;
-    if (ahead.valid()) {
-      n = ahead.get();
-      queued += n;
-      buf.swap(next_buf);
-    }
-  }
-  reader.finish();
-  if (codec_crc) files.close(set_crc);
-  std::vector<uint8_t> tbl;
-  put32(tbl, (uint32_t)es.size());
-  for (const Entry& e : es) {
-    tbl.insert(tbl.end(), {(uint8_t)(e.dir ? 1 : 0), 0, 0, 0});
-    put32(tbl, (uint32_t)e.name.size());
-    tbl.insert(tbl.end(), e.name.begin(), e.name.end());
-    put64(tbl, e.size);
-    put64(tbl, (uint64_t)e.mtime);
-    put32(tbl, e.attr);
-    put32(tbl, e.crc);
-  }
-  std::vector<uint8_t> foot;
-  put64(foot, 24 + st.compressed + st.blocks * 16);
-  put64(foot, tbl.size());
-  put32(foot, crc::of(tbl.data(), tbl.size()));
-  foot.insert(foot.end(), {'Z', 'E', 'N', 'D'});
-  fwrite(tbl.data(), 1, tbl.size(), f);
-  fwrite(foot.data(), 1, foot.size(), f);
-  if (fflush(f) != 0 || ferror(f)) throw Error("writing " + out_path + " failed (disk full?)");
-  fclose(f);
-  closer.f = nullptr;
-  st.seconds = now() - t0;
-  return st;
-}
-
-struct ArchiveInfo {
-  uint32_t block_log = 0;
-  uint64_t total_raw = 0, n_blocks = 0, table_off = 0, file_size = 0;
-  std::vector<Entry> entries;
-  bool codec_file_crcs = false;  // read_blocks checks the files' CRCs with the codec: sinks need not
-};
-
-// header, footer and entry table, every field checked (src/archive.h:452-501)
-inline ArchiveInfo read_info(FILE* f) {
-  ArchiveInfo a;
-  uint8_t h[24];
-  if (fread(h, 1, 24, f) != 24 || memcmp(h, "ZERO", 4) != 0) throw Error("not a .zero archive");
-  if (h[4] != 2) throw Error("archive version " + std::to_string(h[4]) + ": this program reads version 2");
-  a.block_log = h[5];
-  a.total_raw = get64(h + 8);
-  a.n_blocks = get64(h + 16);
-  if (a.block_log < MIN_BLOCK_LOG || a.block_log > MAX_BLOCK_LOG) throw Error("archive damaged: block size");
-  if (fseeko(f, 0, SEEK_END) != 0) throw Error("cannot seek");
-  const int64_t size = (int64_t)ftello(f);
-  if (size < 48) throw Error("archive damaged: too short");
-  a.file_size = (uint64_t)size;
-  uint8_t ft[24];
-  fseeko(f, size - 24, SEEK_SET);
-  if (fread(ft, 1, 24, f) != 24 || memcmp(ft + 20, "ZEND", 4) != 0) throw Error("archive damaged: footer (the file was cut short?)");
-  a.table_off = get64(ft);
-  const uint64_t tl = get64(ft + 8);
-  const uint32_t tc = get32(ft + 16);
-  if (a.table_off + tl + 24 != a.file_size) throw Error("archive damaged: table position");
-  std::vector<uint8_t> tbl(tl);
-  fseeko(f, (off_t)a.table_off, SEEK_SET);
-  if (fread(tbl.data(), 1, tl, f) != tl) throw Error("archive damaged: table cut");
-  if (crc::of(tbl.data(), tl) != tc) throw Error("archive damaged: table checksum");
-  if (tl < 4) throw Error("archive damaged: table");
-  const uint8_t* p = tbl.data();
-  const uint8_t* end = p + tl;
-  const uint32_t n = get32(p);
-  p += 4;
-  uint64_t sum = 0;
-  for (uint32_t i = 0; i < n; ++i) {
-    if (end - p < 8) throw Error("archive damaged: entry");
-    Entry e;
-    e.dir = p[0] == 1;
-    const uint32_t nl = get32(p + 4);
-    p += 8;
-    if ((size_t)(end - p) < (size_t)nl + 24) throw Error("archive damaged: entry name");
-    e.name.assign((const char*)p, nl);
-    p += nl;
-    e.size = get64(p);
-    e.mtime = (int64_t)get64(p + 8);
-    e.attr = get32(p + 16);
-    e.crc = get32(p + 20);
-    p += 24;
-    if (!e.dir) sum += e.size;
-    a.entries.push_back(std::move(e));
-  }
-  if (sum != a.total_raw) throw Error("archive damaged: sizes");
-  return a;
-}
-
-inline ArchiveInfo list_archive(const std::string& path) {
-  FILE* f = fopen(path.c_str(), "rb");
-  if (!f) throw Error("cannot open " + path);
-  struct Closer { FILE* f; ~Closer() { fclose(f); } } c{f};
-  return read_info(f);
-}
-
-// a name from an archive is written only under the target folder (src/archive.h:510-522)
-inline bool safe_name(const std::string& n) {
-  if (n.empty() || n[0] == '/' || n.find('\\') != std::string::npos || n.find(':') != std::string::npos) return false;
-  size_t i = 0;
-  while (i <= n.size()) {
-    size_t j = n.find('/', i);
-    if (j == std::string::npos) j = n.size();
-    const std::string part = n.substr(i, j - i);
-    if (part.empty() || part == "." || part == "..") return false;
-    i = j + 1;
-  }
-  return true;
-}
-
 struct ExtractStats {
   uint64_t raw = 0, compressed = 0, blocks = 0;
   size_t files = 0, folders = 0;
   double seconds = 0, codec_seconds = 0;
+  double read_seconds = 0, check_seconds = 0, write_seconds = 0;  // (worker threads' busy time)
 };
 
-// every block decompressed in order, its CRC checked, and the solid stream handed to sink (src/archive.h:536-639):
-// the next batch is read ahead while this one decodes, and the one before is checked and sunk meanwhile
-inline ExtractStats read_blocks(const std::string& path, Codec& codec, Report& rep, ArchiveInfo& a,
-                                const std::function<void(const uint8_t*, size_t)>& sink) {
-  const double t0 = now();
-  ExtractStats st;
-  FILE* f = fopen(path.c_str(), "rb");
-  if (!f) throw Error("cannot open " + path);
-  struct Closer { FILE* f; ~Closer() { fclose(f); } } c{f};
-  a = read_info(f);
-  codec.prepare(a.block_log);
-  std::vector<uint32_t> probe;
-  a.codec_file_crcs = codec.range_crcs(false, {}, {}, probe);  // (the files' CRCs on the GPU when it can)
-  FileCrcs files(a.entries);
-  auto check_crc = [&a](size_t i, uint32_t c) {
-    if (a.entries[i].size > 0 && c != a.entries[i].crc)
-      throw Error("file " + a.entries[i].name + " does not match its checksum");
-  };
-  fseeko(f, 24, SEEK_SET);
-  const size_t B = (size_t)1 << a.block_log;
-  const size_t per_batch = std::max(B, codec.batch_bytes_decode(a.block_log) / B * B) / B;
-  struct Batch {
-    std::vector<uint8_t> comp;
-    std::vector<BlockIn> ins;
-    std::vector<uint32_t> crcs;
-    size_t raw_total = 0;
-  };
-  uint64_t done = 0, unread = a.n_blocks, undecoded = a.n_blocks, pos = 24;
-  auto read_batch = [&](Batch& t, size_t nb) {
-    t.comp.clear();
-    t.ins.clear();
-    t.crcs.clear();
-    t.raw_total = 0;
-    std::vector<size_t> offs;
-    for (size_t b = 0; b < nb; ++b) {
-      uint8_t h[16];
-      if (fread(h, 1, 16, f) != 16) throw Error("archive damaged: block header cut");
-      pos += 16;
-      BlockIn in;
-      in.raw_len = get32(h);
-      in.comp_len = get32(h + 4);
-      in.method = h[8];
-      if (in.raw_len > B || (in.method == 0 && in.comp_len != in.raw_len) || in.method > 1) throw Error("archive damaged: block header");
-      if (pos + in.comp_len > a.table_off) throw Error("archive damaged: block past the table");
-      offs.push_back(t.comp.size());
-      t.comp.resize(t.comp.size() + in.comp_len);
-      if (fread(t.comp.data() + offs.back(), 1, in.comp_len, f) != in.comp_len) throw Error("archive damaged: block cut");
-      pos += in.comp_len;
-      t.crcs.push_back(get32(h + 12));
-      t.raw_total += in.raw_len;
-      t.ins.push_back(in);
+// a thread that runs tasks in the order they are given
+class Worker {
+ public:
+  Worker() : t_([this] { loop(); }) {}
+  ~Worker() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
     }
-    for (size_t b = 0; b < nb; ++b) t.ins[b].data = t.comp.data() + offs[b];
-  };
-  rep.at(0, a.total_raw);
-  Batch batches[2];
-  int at = 0;
-  std::future<void> reading, writing;
-  auto take = [&](Batch& t) {
-    const size_t nb = (size_t)std::min<uint64_t>(per_batch, unread);
-    unread -= nb;
-    return std::async(std::launch::async, read_batch, std::ref(t), nb);
-  };
-  if (unread) reading = take(batches[at]);
-  while (undecoded > 0) {
-    if (rep.cancelled()) throw Cancelled();
-    reading.get();
-    if (unread) reading = take(batches[at ^ 1]);
-    Batch& cur = batches[at];
-    const size_t nb = cur.ins.size();
-    uint64_t comp_bytes = 0;
-    for (const BlockIn& k : cur.ins) comp_bytes += k.comp_len;
-    const double c0 = now();
-    const uint8_t* raw = codec.decompress(cur.ins);
-    st.codec_seconds += now() - c0;
-    st.compressed += comp_bytes;
-    if (writing.valid()) writing.get();  // (the batch before: its bytes stay valid until this call's next)
-    std::vector<uint32_t> got(nb);
-    if (const uint32_t* gc = codec.block_crcs()) std::copy(gc, gc + nb, got.begin());
-    else {
-      size_t o = 0;
-      for (size_t b = 0; b < nb; ++b) { got[b] = crc::of(raw + o, cur.ins[b].raw_len); o += cur.ins[b].raw_len; }
-    }
-    for (size_t b = 0; b < nb; ++b)
-      if (got[b] != cur.crcs[b])
-        throw Error("block " + std::to_string(st.blocks + b) + " does not match its checksum: the archive is damaged");
-    if (a.codec_file_crcs) files.batch(codec, false, cur.raw_total, check_crc);
-    const size_t raw_total = cur.raw_total;
-    writing = std::async(std::launch::async, [&, raw, raw_total, nb] {
-      sink(raw, raw_total);
-      done += raw_total;
-      st.blocks += nb;
-      rep.at(done, a.total_raw);
-    });
-    undecoded -= nb;
-    at ^= 1;
+    cv_.notify_one();
+    t_.join();
   }
-  if (writing.valid()) writing.get();
-  if (done != a.total_raw) throw Error("archive damaged: the blocks hold " + std::to_string(done) + " bytes, the table " + std::to_string(a.total_raw));
-  if (a.codec_file_crcs) files.close(check_crc);
-  st.raw = done;
+  std::future<void> run(std::function<void()> f) {
+    std::packaged_task<void()> task(std::move(f));
+    std::future<void> fut = task.get_future();
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      q_.push_back(std::move(task));
+    }
+    cv_.notify_one();
+    return fut;
+  }
+
+ private:
+  void loop() {
+    for (;;) {
+      std::packaged_task<void()> task;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [this] { return stop_ || !q_.empty(); });
+        if (q_.empty()) return;
+        task = std::move(q_.front());
+        q_.pop_front();
+      }
+      task();
+    }
+  }
+  std::mutex mu_;
+  std::condition_variable cv_;
+  std::deque<std::packaged_task<void()>> q_;
+  bool stop_ = false;
+  std::thread t_;
+};
+
+// waits for every future that is still pending (after a failure, so nothing runs on after the pipeline is gone)
+inline void drain(std::vector<std::future<void>*> fs) {
+  for (std::future<void>* f : fs)
+    if (f->valid()) {
+      try { f->get(); } catch (...) {}
+    }
+}
+
+// host memory for a batch: the codec's (pinned) when it has some, else our own
+struct SlotMemory {
+  std::vector<uint8_t> own;
+  uint8_t* p = nullptr;
+  uint8_t* get(Codec& c, size_t bytes, int slot) {
+    if (uint8_t* q = c.batch_memory(bytes, slot)) return p = q;
+    if (own.size() < bytes) own.resize(bytes);
+    return p = own.data();
+  }
+};
+
+// the archive being written: appended through a large buffer; removed unless committed (an archive left
+// unfinished by an error or a cancel could only fail later, src/archive.h:347-352)
+class ArchiveFile {
+ public:
+  explicit ArchiveFile(const std::string& path) : path_(path) {
+    fd_ = ::open(path.c_str(), O_WRONLY | O_CREAT | O_TRUNC | O_CLOEXEC, 0644);
+    if (fd_ < 0) throw Error("cannot create " + path);
+    buf_.reserve(CAP);
+  }
+  ~ArchiveFile() {
+    if (fd_ >= 0) {
+      ::close(fd_);
+      ::unlink(path_.c_str());
+    }
+  }
+  void put(const void* p, size_t n) {
+    const uint8_t* s = (const uint8_t*)p;
+    if (buf_.size() + n > CAP) flush();
+    if (n >= CAP) { write_all(s, n); return; }
+    buf_.insert(buf_.end(), s, s + n);
+  }
+  void commit() {
+    flush();
+    if (::close(fd_) != 0) throw Error("writing " + path_ + " failed (disk full?)");
+    fd_ = -1;
+  }
+
+ private:
+  static constexpr size_t CAP = (size_t)8 << 20;
+  void flush() {
+    write_all(buf_.data(), buf_.size());
+    buf_.clear();
+  }
+  void write_all(const uint8_t* p, size_t n) {
+    while (n) {
+      const ssize_t w = ::write(fd_, p, n);
+      if (w < 0 && errno == EINTR) continue;
+      if (w <= 0) throw Error("writing " + path_ + " failed (disk full?)");
+      p += w;
+      n -= (size_t)w;
+    }
+  }
+  std::string path_;
+  int fd_ = -1;
+  std::vector<uint8_t> buf_;
+};
+
+// the pieces of stream range [a, a + n) as offsets within that range (for Codec::range_crcs), and their entries
+struct Pieces {
+  std::vector<uint64_t> off, len;
+  std::vector<size_t> entry;
+  void of(const Layout& lay, uint64_t a, uint64_t n) {
+    off.clear(), len.clear(), entry.clear();
+    lay.pieces(a, n, [&](size_t i, uint64_t o, uint64_t l) {
+      off.push_back(lay.start(i) + o - a);
+      len.push_back(l);
+      entry.push_back(i);
+    });
+  }
+};
+
+// ---------------------------------------------------------------- compress
+inline CompressStats compress_archive(const std::vector<std::string>& inputs, const std::string& out_path, Codec& codec,
+                                      uint32_t block_log, Report& rep) {
+  if (block_log < fmt::MIN_BLOCK_LOG || block_log > fmt::MAX_BLOCK_LOG)
+    throw Error("block size must be 64 KiB .. 1 MiB (log2 16..20)");
+  const double t0 = now();
+  CompressStats st;
+  const std::vector<Source> src = scan_inputs(inputs, [&](const std::string& s) { rep.say(s); });
+  std::vector<fmt::Item> items;
+  for (const Source& s : src) {
+    items.push_back(s.item);
+    if (s.item.folder) ++st.folders; else ++st.files;
+  }
+  const Layout lay(items);
+  const uint64_t total = lay.total();
+  st.raw = total;
+  rep.say(std::to_string(st.files) + " files, " + std::to_string(st.folders) + " folders, " + std::to_string(total) +
+          " bytes");
+  const uint64_t B = 1ull << block_log;
+  ArchiveFile arc(out_path);
+  fmt::Header hdr;
+  hdr.block_log = block_log;
+  hdr.raw_bytes = total;
+  hdr.blocks = (total + B - 1) / B;
+  uint8_t h[fmt::HEADER_BYTES];
+  fmt::put(hdr, h);
+  arc.put(h, sizeof h);
+
+  const uint64_t batch = std::max<uint64_t>(B, codec.batch_bytes(block_log) / B * B);
+  const uint64_t nbatch = (total + batch - 1) / batch;
+  std::vector<uint32_t> probe;
+  const bool codec_file_sums = codec.range_crcs(true, {}, {}, probe);
+  codec.hand_out_views(true);
+  FileSums sums(items.size());
+  StreamReader reader(src, lay);
+  struct Slot {
+    SlotMemory mem;
+    uint64_t at = 0;
+    size_t n = 0;
+    std::vector<BlockOut> outs;
+    std::vector<uint32_t> crcs;
+    Pieces pieces;
+    std::vector<uint32_t> piece_crcs;  // (empty: computed on the host when written)
+  } slot[3];
+  Worker disk_in, disk_out;
+  std::future<void> reading[3], writing[3];
+  auto start_read = [&](uint64_t k) {
+    Slot& s = slot[k % 3];
+    s.at = k * batch;
+    s.n = (size_t)std::min<uint64_t>(batch, total - s.at);
+    uint8_t* m = s.mem.get(codec, (size_t)batch, (int)(k % 3));
+    reading[k % 3] = disk_in.run([&reader, &s, m] { reader.read(s.at, m, s.n); });
+  };
+  auto write_batch = [&](Slot& s) {  // (the archive's blocks go out in batch order: one writer)
+    const uint8_t* in = s.mem.p;
+    for (size_t b = 0; b < s.outs.size(); ++b) {
+      const size_t off = b * B;
+      const uint32_t len = (uint32_t)std::min<uint64_t>(B, s.n - off);
+      const BlockOut& o = s.outs[b];
+      const bool stored = o.method == 0 || o.size() >= len;
+      fmt::Record r;
+      r.raw_len = len;
+      r.comp_len = stored ? len : (uint32_t)o.size();
+      r.method = stored ? fmt::METHOD_STORED : o.method;
+      r.crc = s.crcs.empty() ? crc::of(in + off, len) : s.crcs[b];
+      uint8_t rb[fmt::RECORD_BYTES];
+      fmt::put(r, rb);
+      arc.put(rb, sizeof rb);
+      arc.put(stored ? in + off : o.bytes(), r.comp_len);
+      st.compressed += r.comp_len;
+      st.stored_blocks += stored;
+      ++st.blocks;
+    }
+    for (size_t i = 0; i < s.pieces.off.size(); ++i)
+      sums.add(s.pieces.entry[i], s.piece_crcs.empty() ? crc::of(in + s.pieces.off[i], s.pieces.len[i]) : s.piece_crcs[i],
+               s.pieces.len[i]);
+    rep.at(s.at + s.n, total);
+  };
+  rep.at(0, total);
+  try {
+    if (nbatch) start_read(0);
+    for (uint64_t k = 0; k < nbatch; ++k) {
+      if (rep.cancelled()) throw Cancelled();
+      Slot& s = slot[k % 3];
+      reading[k % 3].get();
+      // batch k - 2 is written: its slot is free, and so are the codec buffers it used (this call reuses them)
+      if (writing[(k + 1) % 3].valid()) writing[(k + 1) % 3].get();
+      if (k + 1 < nbatch) start_read(k + 1);
+      const double c0 = now();
+      codec.compress(s.mem.p, s.n, block_log, s.outs);
+      const uint32_t* bc = codec.block_crcs();
+      s.crcs.assign(bc ? bc : nullptr, bc ? bc + s.outs.size() : nullptr);
+      s.pieces.of(lay, s.at, s.n);
+      s.piece_crcs.clear();
+      if (codec_file_sums && !s.pieces.off.empty() && !codec.range_crcs(true, s.pieces.off, s.pieces.len, s.piece_crcs))
+        throw Error("the codec lost its checksum support");
+      st.codec_seconds += now() - c0;
+      writing[k % 3] = disk_out.run([&write_batch, &s] { write_batch(s); });
+    }
+    for (auto& w : writing)
+      if (w.valid()) w.get();
+  } catch (...) {
+    drain({&reading[0], &reading[1], &reading[2], &writing[0], &writing[1], &writing[2]});
+    throw;
+  }
+  // the table, its place, and the footer
+  for (size_t i = 0; i < items.size(); ++i)
+    if (!items[i].folder) items[i].crc = items[i].size ? sums.of(i) : 0;
+  const std::vector<uint8_t> tbl = fmt::table_bytes(items);
+  arc.put(tbl.data(), tbl.size());
+  fmt::Footer ft;
+  ft.table_off = fmt::HEADER_BYTES + st.compressed + st.blocks * fmt::RECORD_BYTES;
+  ft.table_len = tbl.size();
+  ft.table_crc = crc::of(tbl.data(), tbl.size());
+  uint8_t fb[fmt::FOOTER_BYTES];
+  fmt::put(ft, fb);
+  arc.put(fb, sizeof fb);
+  arc.commit();
   st.seconds = now() - t0;
   return st;
 }
 
-// the solid stream written back into files under a folder (src/archive.h:642-722)
-struct SolidWriter {
-  const std::vector<Entry>& es;
-  fs::path root;
-  size_t idx = 0;
-  uint64_t pos = 0;
-  FILE* f = nullptr;
-  uint32_t crc = 0;
-  bool host_crc = true;  // (false: read_blocks checks the files' CRCs)
-  size_t files = 0, folders = 0;
-  SolidWriter(const std::vector<Entry>& e, const fs::path& r) : es(e), root(r) {}
-  ~SolidWriter() { if (f) fclose(f); }
-  void open_next() {
-    const Entry& e = es[idx];
-    if (!safe_name(e.name)) throw Error("refused: unsafe name in the archive: " + e.name);
-    const fs::path p = root / fs::path(e.name);
-    std::error_code ec;
-    fs::create_directories(p.parent_path(), ec);
-    if (!(f = fopen(p.c_str(), "wb"))) throw Error("cannot create " + p.string());
-    ++files;
-  }
-  void finish_file() {
-    const Entry& e = es[idx];
-    if (f) { fclose(f); f = nullptr; }
-    if (host_crc && !e.dir && crc != e.crc && e.size > 0) throw Error("file " + e.name + " does not match its checksum");
-    set_mtime(root / fs::path(e.name), e.mtime);
-    crc = 0;
-  }
-  void settle() {
-    while (idx < es.size()) {
-      const Entry& e = es[idx];
-      if (e.dir) {
-        if (!safe_name(e.name)) throw Error("refused: unsafe name in the archive: " + e.name);
-        std::error_code ec;
-        fs::create_directories(root / fs::path(e.name), ec);
-        ++folders;
-        ++idx;
-        continue;
-      }
-      if (e.size == 0) { open_next(); finish_file(); ++idx; continue; }
-      if (pos >= e.size) { finish_file(); ++idx; pos = 0; continue; }
-      return;
-    }
-  }
-  void write(const uint8_t* p, size_t n) {
-    settle();
-    while (n > 0) {
-      if (idx >= es.size()) throw Error("archive damaged: more data than the table lists");
-      const Entry& e = es[idx];
-      if (!f) open_next();
-      const size_t want = (size_t)std::min<uint64_t>(n, e.size - pos);
-      if (fwrite(p, 1, want, f) != want) throw Error("writing " + e.name + " failed (disk full?)");
-      if (host_crc) crc = crc::update(crc, p, want);
-      p += want;
-      n -= want;
-      pos += want;
-      settle();
-    }
-  }
-  void close() {
-    settle();
-    for (const Entry& e : es)
-      if (e.dir) set_mtime(root / fs::path(e.name), e.mtime);
-  }
+// ---------------------------------------------------------------- reading
+struct ArchiveInfo {
+  fmt::Header hdr;
+  fmt::Footer foot;
+  uint64_t file_size = 0;
+  std::vector<fmt::Item> entries;
 };
 
-inline ExtractStats extract_archive(const std::string& path, const std::string& out_dir, Codec& codec, Report& rep) {
-  ArchiveInfo a;
-  const fs::path root(out_dir);
-  std::error_code ec;
-  fs::create_directories(root, ec);
-  std::unique_ptr<SolidWriter> w;
-  ExtractStats st = read_blocks(path, codec, rep, a, [&](const uint8_t* p, size_t n) {
-    if (!w) { w = std::make_unique<SolidWriter>(a.entries, root); w->host_crc = !a.codec_file_crcs; }
-    w->write(p, n);
-  });
-  if (!w) w = std::make_unique<SolidWriter>(a.entries, root);
-  w->close();
-  st.files = w->files;
-  st.folders = w->folders;
+// an archive opened for reading, with positional reads
+class ArchiveIn {
+ public:
+  explicit ArchiveIn(const std::string& path) : path_(path) {
+    fd_ = ::open(path.c_str(), O_RDONLY | O_CLOEXEC);
+    if (fd_ < 0) throw Error("cannot open " + path);
+  }
+  ~ArchiveIn() { ::close(fd_); }
+  void read(uint64_t at, void* p, size_t n, const char* what) const {
+    size_t got = 0;
+    while (got < n) {
+      const ssize_t r = ::pread(fd_, (uint8_t*)p + got, n - got, (off_t)(at + got));
+      if (r < 0 && errno == EINTR) continue;
+      if (r <= 0) fmt::damaged(what);
+      got += (size_t)r;
+    }
+  }
+  // header, footer and table, each checked
+  ArchiveInfo info() const {
+    ArchiveInfo a;
+    struct stat sb;
+    if (fstat(fd_, &sb) != 0) throw Error("cannot read " + path_);
+    a.file_size = (uint64_t)sb.st_size;
+    uint8_t h[fmt::HEADER_BYTES];
+    if (a.file_size < fmt::HEADER_BYTES) throw Error("not a .zero archive");
+    read(0, h, sizeof h, "header");
+    a.hdr = fmt::get_header(h);
+    if (a.file_size < fmt::HEADER_BYTES + fmt::FOOTER_BYTES) fmt::damaged("too short");
+    uint8_t f[fmt::FOOTER_BYTES];
+    read(a.file_size - fmt::FOOTER_BYTES, f, sizeof f, "footer (the file was cut short?)");
+    a.foot = fmt::get_footer(f);
+    if (a.foot.table_off < fmt::HEADER_BYTES || a.foot.table_len > a.file_size ||
+        a.foot.table_off + a.foot.table_len + fmt::FOOTER_BYTES != a.file_size)
+      fmt::damaged("table position");
+    std::vector<uint8_t> t((size_t)a.foot.table_len);
+    read(a.foot.table_off, t.data(), t.size(), "table cut");
+    if (crc::of(t.data(), t.size()) != a.foot.table_crc) fmt::damaged("table checksum");
+    a.entries = fmt::parse_table(t.data(), t.size(), a.hdr.raw_bytes);
+    return a;
+  }
+
+ private:
+  std::string path_;
+  int fd_ = -1;
+};
+
+inline ArchiveInfo list_archive(const std::string& path) { return ArchiveIn(path).info(); }
+
+// Decodes every block in order and hands the solid stream to sink(stream offset, bytes, n) on the writer thread,
+// batch by batch; every block's CRC and every file's CRC is checked.
+inline ExtractStats decode_archive(const std::string& path, Codec& codec, Report& rep, ArchiveInfo& a,
+                                   const std::function<void(uint64_t, const uint8_t*, size_t)>& sink) {
+  const double t0 = now();
+  ExtractStats st;
+  const ArchiveIn arc(path);
+  a = arc.info();
+  codec.prepare(a.hdr.block_log);
+  const uint64_t B = 1ull << a.hdr.block_log;
+  const uint64_t per = std::max<uint64_t>(1, codec.batch_bytes_decode(a.hdr.block_log) / B);
+  const uint64_t nbatch = (a.hdr.blocks + per - 1) / per;
+  const Layout lay(a.entries);
+  FileSums sums(a.entries.size());
+  std::vector<uint32_t> probe;
+  const bool codec_file_sums = codec.range_crcs(false, {}, {}, probe);
+  struct Slot {
+    SlotMemory mem;
+    std::vector<BlockIn> ins;
+    std::vector<uint32_t> want;  // the records' CRCs
+    uint64_t at = 0, raw = 0, comp = 0, first = 0;
+    const uint8_t* out = nullptr;
+    std::vector<uint32_t> crcs, piece_crcs;
+    Pieces pieces;
+  } slot[3];
+  uint64_t next_pos = fmt::HEADER_BYTES, next_at = 0;  // (the reader's place in the archive and in the stream)
+  Worker disk_in, disk_out;
+  std::future<void> reading[3], writing[3];
+  double t_read = 0, t_check = 0, t_write = 0;
+  auto read_batch = [&](Slot& s, uint64_t first, uint64_t nb) {
+    const double r0 = now();
+    s.ins.assign(nb, BlockIn{});
+    s.want.resize(nb);
+    s.first = first;
+    s.at = next_at;
+    // the records first (16 bytes each, in front of their payloads), then the payloads in one read per block into
+    // the slot's memory, which is at most the batch's raw size plus its records (a payload is smaller than its
+    // block or stored)
+    uint8_t* m = s.mem.get(codec, (size_t)(per * B), (int)(&s - slot));
+    uint64_t pos = next_pos, used = 0, raw = 0;
+    for (uint64_t b = 0; b < nb; ++b) {
+      uint8_t rb[fmt::RECORD_BYTES];
+      if (pos + fmt::RECORD_BYTES > a.foot.table_off) fmt::damaged("block header cut");
+      arc.read(pos, rb, sizeof rb, "block header cut");
+      pos += fmt::RECORD_BYTES;
+      const fmt::Record r = fmt::get_record(rb, a.hdr.block_log);
+      if (r.comp_len > a.foot.table_off - pos) fmt::damaged("block past the table");
+      if (used + r.comp_len > per * B) fmt::damaged("block header");
+      arc.read(pos, m + used, r.comp_len, "block cut");
+      s.ins[b].method = r.method;
+      s.ins[b].raw_len = r.raw_len;
+      s.ins[b].comp_len = r.comp_len;
+      s.ins[b].data = m + used;
+      s.want[b] = r.crc;
+      pos += r.comp_len;
+      used += r.comp_len;
+      raw += r.raw_len;
+    }
+    if (raw > a.hdr.raw_bytes - next_at) fmt::damaged("the blocks hold more bytes than the table lists");
+    s.raw = raw;
+    s.comp = used;
+    next_pos = pos;
+    next_at += raw;
+    t_read += now() - r0;
+  };
+  auto check_and_write = [&](Slot& s) {
+    const double c1 = now();
+    uint64_t o = 0;
+    for (size_t b = 0; b < s.ins.size(); ++b) {
+      const uint32_t got = s.crcs.empty() ? crc::of(s.out + o, s.ins[b].raw_len) : s.crcs[b];
+      if (got != s.want[b])
+        throw Error("block " + std::to_string(s.first + b) + " does not match its checksum: the archive is damaged");
+      o += s.ins[b].raw_len;
+    }
+    for (size_t i = 0; i < s.pieces.off.size(); ++i)
+      sums.add(s.pieces.entry[i],
+               s.piece_crcs.empty() ? crc::of(s.out + s.pieces.off[i], s.pieces.len[i]) : s.piece_crcs[i],
+               s.pieces.len[i]);
+    const double c2 = now();
+    sink(s.at, s.out, (size_t)s.raw);
+    t_check += c2 - c1;
+    t_write += now() - c2;
+    st.blocks += s.ins.size();
+    st.raw += s.raw;
+    rep.at(st.raw, a.hdr.raw_bytes);
+  };
+  rep.at(0, a.hdr.raw_bytes);
+  auto start_read = [&](uint64_t k) {
+    Slot& s = slot[k % 3];
+    const uint64_t first = k * per, nb = std::min<uint64_t>(per, a.hdr.blocks - first);
+    reading[k % 3] = disk_in.run([&read_batch, &s, first, nb] { read_batch(s, first, nb); });
+  };
+  try {
+    if (nbatch) start_read(0);
+    for (uint64_t k = 0; k < nbatch; ++k) {
+      if (rep.cancelled()) throw Cancelled();
+      Slot& s = slot[k % 3];
+      reading[k % 3].get();
+      if (writing[(k + 1) % 3].valid()) writing[(k + 1) % 3].get();  // (as in compress_archive)
+      if (k + 1 < nbatch) start_read(k + 1);
+      const double c0 = now();
+      s.out = codec.decompress(s.ins);
+      const uint32_t* bc = codec.block_crcs();
+      s.crcs.assign(bc ? bc : nullptr, bc ? bc + s.ins.size() : nullptr);
+      s.pieces.of(lay, s.at, s.raw);
+      s.piece_crcs.clear();
+      if (codec_file_sums && !s.pieces.off.empty() && !codec.range_crcs(false, s.pieces.off, s.pieces.len, s.piece_crcs))
+        throw Error("the codec lost its checksum support");
+      st.codec_seconds += now() - c0;
+      st.compressed += s.comp;
+      writing[k % 3] = disk_out.run([&check_and_write, &s] { check_and_write(s); });
+    }
+    for (auto& w : writing)
+      if (w.valid()) w.get();
+  } catch (...) {
+    drain({&reading[0], &reading[1], &reading[2], &writing[0], &writing[1], &writing[2]});
+    throw;
+  }
+  if (next_pos != a.foot.table_off) fmt::damaged("the blocks do not end where the table starts");
+  if (st.raw != a.hdr.raw_bytes)
+    fmt::damaged("the blocks hold " + std::to_string(st.raw) + " bytes, the table " + std::to_string(a.hdr.raw_bytes));
+  for (size_t i = 0; i < a.entries.size(); ++i) {
+    const fmt::Item& e = a.entries[i];
+    if (!e.folder && e.size > 0 && sums.of(i) != e.crc) throw Error("file " + e.name + " does not match its checksum");
+  }
+  st.read_seconds = t_read;
+  st.check_seconds = t_check;
+  st.write_seconds = t_write;
+  st.seconds = now() - t0;
   return st;
 }
 
-// everything decompressed and every block and file checksum checked, nothing written (src/archive.h:747-785)
+inline ExtractStats extract_archive(const std::string& path, const std::string& out_dir, Codec& codec, Report& rep) {
+  ArchiveInfo a;
+  std::unique_ptr<Layout> lay;
+  std::unique_ptr<StreamWriter> w;
+  ExtractStats st = decode_archive(path, codec, rep, a, [&](uint64_t at, const uint8_t* p, size_t n) {
+    if (!w) {
+      lay = std::make_unique<Layout>(a.entries);
+      w = std::make_unique<StreamWriter>(a.entries, *lay, out_dir);
+    }
+    w->write(at, p, n);
+  });
+  if (!w) {
+    lay = std::make_unique<Layout>(a.entries);
+    w = std::make_unique<StreamWriter>(a.entries, *lay, out_dir);
+  }
+  w->finish();
+  st.files = w->files();
+  st.folders = w->folders();
+  if (getenv("ZERO_PROFILE"))
+    fprintf(stderr, "[profile] extract (s): read %.3f codec %.3f check %.3f write %.3f, whole %.3f\n", st.read_seconds,
+            st.codec_seconds, st.check_seconds, st.write_seconds, st.seconds);
+  return st;
+}
+
+// decodes everything and checks every block's and every file's checksum, writing nothing
 inline ExtractStats test_archive(const std::string& path, Codec& codec, Report& rep) {
   ArchiveInfo a;
-  size_t idx = 0, files = 0;
-  uint64_t pos = 0;
-  uint32_t crc = 0;
-  auto settle = [&] {
-    while (idx < a.entries.size()) {
-      const Entry& e = a.entries[idx];
-      if (e.dir) { ++idx; continue; }
-      if (pos >= e.size) {
-        if (!a.codec_file_crcs && e.size > 0 && crc != e.crc) throw Error("file " + e.name + " does not match its checksum");
-        ++files;
-        ++idx;
-        pos = 0;
-        crc = 0;
-        continue;
-      }
-      return;
-    }
-  };
-  ExtractStats st = read_blocks(path, codec, rep, a, [&](const uint8_t* p, size_t n) {
-    settle();
-    while (n > 0) {
-      if (idx >= a.entries.size()) throw Error("archive damaged: more data than the table lists");
-      const Entry& e = a.entries[idx];
-      const size_t want = (size_t)std::min<uint64_t>(n, e.size - pos);
-      if (!a.codec_file_crcs) crc = crc::update(crc, p, want);
-      p += want;
-      n -= want;
-      pos += want;
-      settle();
-    }
-  });
-  settle();
-  st.files = files;
-  for (const Entry& e : a.entries)
-    if (e.dir) ++st.folders;
+  ExtractStats st = decode_archive(path, codec, rep, a, [](uint64_t, const uint8_t*, size_t) {});
+  for (const fmt::Item& e : a.entries) {
+    if (e.folder) ++st.folders; else ++st.files;
+  }
   return st;
 }
 

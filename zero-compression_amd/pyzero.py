@@ -112,7 +112,7 @@ def load_library(path=LIB_PATH):
         "zgpu_crc32_blocks": (i32, [vp, vp, vp, u32, i32, vp, vp]),
         "zgpu_encode_scratch_bytes": (ctypes.c_size_t, [u32, u32]),
         "zgpu_decode_scratch_bytes": (ctypes.c_size_t, [u32, u32]),
-        "zgpu_compress": (i32, [vp, vp, u64, u32, i32, vp, vp, vp]),
+        "zgpu_compress": (i32, [vp, vp, u64, u32, i32, vp, u64, vp, vp]),
         "zgpu_decompress": (i32, [vp, vp, vp, vp, vp, u32, u32, vp, vp, vp]),
         "zgpu_encode_stream": (i32, [vp, vp, u32, vp, ctypes.POINTER(u32)]),
         "zgpu_crc32": (i32, [vp, vp, u64, u32, vp]),
@@ -234,15 +234,17 @@ class Gpu:
         nb = (n + B - 1) // B
         if nb == 0:
             return []
-        out = np.empty(nb * B, dtype=np.uint8)
+        out = np.empty(max(1, n), dtype=np.uint8)
         lens = np.empty(nb, dtype=np.uint32)
-        self._check(self.lib.zgpu_compress(self.h, _ptr(a), n, block_log, level, _ptr(out), _ptr(lens), None))
-        res = []
+        self._check(self.lib.zgpu_compress(self.h, _ptr(a), n, block_log, level, _ptr(out), out.size, _ptr(lens),
+                                           None))
+        res, o = [], 0
         for b in range(nb):
             if lens[b] == STORED:
                 res.append((0, b""))
             else:
-                res.append((1, out[b * B: b * B + int(lens[b])].tobytes()))
+                res.append((1, out[o: o + int(lens[b])].tobytes()))
+                o += int(lens[b])
         return res
 
     def decompress(self, blocks, block_log):
