@@ -13,7 +13,8 @@ from the reference's headers) on a bounded sample of the same input, on this mac
 
 ``--workload uneven`` measures the batched form instead: 200,000 files with lognormal sizes (median 8 KiB, sigma 2,
 clipped to 64 B .. 64 MiB) cut from the seed-1 LZ stream, level 5, 2^18 blocks, through one size-class plan
-(autotuned, replayed as a hipGraph), against the plain path called once per file on a sample of the same files.
+(persistent stage kernels, autotuned, replayed as a hipGraph), against the plain path called once per file on every
+file: GB/s of both, and p50/p99 latency of a backlog of all files submitted at once.
 
 ``--workload serve`` runs the serving loop under an open-loop load: Poisson arrivals at 20 GB/s of raw bytes for one
 second, 70 % compress / 30 % decompress, sizes lognormal (median 256 KiB, sigma 1.5, at most 256 MiB), seed-1 LZ and
@@ -40,32 +41,61 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(data, bl, depth, budget_s=12.0):
-    """The reference CpuCodec (oracle/_ref/libzeroref.so) on a sample of the input, all cores of this process."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
+def time_ref(ref, data, bl, depth, threads, budget_s):
+    """the reference CpuCodec's compress and decompress of data, each repeated to about budget_s / 2 of wall time;
+    the round trip is checked"""
+    reps_c, tc = 0, 0.0
+    while reps_c < 1 or (tc < budget_s / 2 and reps_c < 50):
+        t0 = time.perf_counter()
+        blob, clen, meth = ref.cpu_compress(data, bl, depth, threads)
+        tc += time.perf_counter() - t0
+        reps_c += 1
+    raw_len = np.array([min(1 << bl, data.size - o) for o in range(0, data.size, 1 << bl)], dtype=np.uint32)
+    reps_d, td = 0, 0.0
+    while reps_d < 3 or (td < budget_s / 2 and reps_d < 200):
+        t0 = time.perf_counter()
+        rc, back = ref.cpu_decompress(blob, clen, meth, raw_len, threads)
+        td += time.perf_counter() - t0
+        reps_d += 1
+        assert rc == 0
+    assert np.array_equal(back, data)
+    return (data.size * reps_c / tc / 1e9, data.size * reps_d / td / 1e9, float(clen.sum()) / data.size, reps_c,
+            reps_d)
+
+
+def cpu_baseline(data, bl, depth, budget_s=8.0):
+    """The reference's own CpuCodec (oracle/_ref/libzeroref.so, compiled from its headers) on this machine's cores:
+    the C1 configuration as stated (64 MiB of the seed-0 LZ stream, level 5 = hash-chain depth 32, 2^18 blocks,
+    `zero bench --cpu`'s work), and beside it a 256 MiB sample of this run's own input."""
+    import pyzero
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import Ref, ref_available
     if not ref_available():
         return None
     ref = Ref()
     threads = len(os.sched_getaffinity(0))
-    sample = data[: 256 << 20]
-    t0 = time.perf_counter()
-    blob, clen, meth = ref.cpu_compress(sample, bl, depth, threads)
-    tc = time.perf_counter() - t0
-    raw_len = np.array([min(1 << bl, sample.size - o) for o in range(0, sample.size, 1 << bl)], dtype=np.uint32)
-    reps, td = 0, 0.0
-    while reps < 3 or (td < budget_s / 3 and reps < 20):
-        t0 = time.perf_counter()
-        rc, back = ref.cpu_decompress(blob, clen, meth, raw_len, threads)
-        td += time.perf_counter() - t0
-        reps += 1
-        assert rc == 0
-    assert np.array_equal(back, sample)
+    c1 = pyzero.gen_lz(0, 64 << 20)
+    c_gbps, d_gbps, ratio, rc_, rd_ = time_ref(ref, c1, 18, 32, threads, budget_s)
+    s_c, s_d, s_ratio, _, _ = time_ref(ref, data[: 256 << 20], bl, depth, threads, budget_s / 2)
     return {
-        "value": round(sample.size * reps / td / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": "reference",
-        "sample": f"first {sample.size >> 20} MiB of the seed-1 LZ input, level 5 (chain depth {depth}), 2^{bl} blocks: "
-                  f"decode x{reps} (value), encode once",
-        "compress_GBps": round(sample.size / tc / 1e9, 4), "ratio": round(float(clen.sum()) / sample.size, 4),
+        "value": round(d_gbps, 3), "unit": "GB/s", "cores": threads, "kind": "reference",
+        "cpu": cpu_model(),
+        "sample": f"C1: 64 MiB of the seed-0 LZ stream, level 5 (chain depth 32), 2^18 blocks, all {threads} threads: "
+                  f"decode x{rd_} (value), encode x{rc_}; round trip identical",
+        "compress_GBps": round(c_gbps, 4), "ratio": round(ratio, 5),
+        "same_workload_sample": {"input": f"first 256 MiB of this run's seed-1 input, depth {depth}, 2^{bl} blocks",
+                                 "decompress_GBps": round(s_d, 3), "compress_GBps": round(s_c, 4),
+                                 "ratio": round(s_ratio, 5)},
     }
 
 
@@ -111,7 +141,7 @@ def bench_uneven(a, gpu):
         gpu.sync()
         return gpu.elapsed_ms(ev0, ev1) / steps
 
-    plan = pyzero.Plan(gpu, src, rl, bl, a.streams)
+    plan = pyzero.Plan(gpu, src, rl, bl)
     d_slabs = gpu.alloc(plan.slab_bytes)
     d_len = gpu.alloc(nb * 4)
     t0 = time.time()
@@ -142,70 +172,79 @@ def bench_uneven(a, gpu):
     log(f"uneven batched: encode {total / enc_ms / 1e6:.1f} GB/s, decode {total / dec_ms / 1e6:.1f} GB/s, ratio "
         f"{comp.size / total:.4f}, round trip {'identical' if ok else 'MISMATCH'}")
 
-    # the plain path, one call per file, on the first files (device-resident, same buffers)
-    nfs = min(a.files, a.sample_files)
-    nbs = int(np.searchsorted(fi, nfs))
-    sample_bytes = int(rl[:nbs].sum())
-    tab = pyzero.block_table(rl[:nbs], src=src[:nbs], dst=off[:nbs])
+    # the plain path, one call per file, over every file: the 200k files are a backlog submitted at t = 0, and a
+    # file's latency is the time until its results are on the device; an event after every 64th file's call times
+    # the files of each group of 64 (the last file of a group exactly, the others at most 63 files late)
+    nfs = a.files
+    firsts = np.searchsorted(fi, np.arange(nfs + 1))
+    maxb = int(np.max(np.diff(firsts))) if nfs else 1
+    tab = pyzero.block_table(rl, src=src, dst=np.zeros(nb, dtype=np.uint64))
     d_tab = gpu.alloc(max(1, tab.nbytes))
     d_tab.upload(tab)
-    dtab = pyzero.block_table(rl[:nbs], comp_lens=clen[:nbs], methods=np.where(stored[:nbs], 0, 1),
-                              src=coff[:nbs], dst=src[:nbs])
+    dtab = pyzero.block_table(rl, comp_lens=clen, methods=np.where(stored, 0, 1), src=coff, dst=src)
     d_dtab = gpu.alloc(max(1, dtab.nbytes))
     d_dtab.upload(dtab)
-    firsts = np.searchsorted(fi[:nbs], np.arange(nfs + 1))
-    d_pslab = gpu.alloc(max(1, nbs) * B)
-    d_plen = gpu.alloc(max(1, nbs) * 4)
+    d_pslab = gpu.alloc(max(1, maxb) * B)  # (one file's slabs at a time)
+    d_plen = gpu.alloc(max(1, nb) * 4)
+    G = 64
+    evs = [gpu.event() for _ in range((nfs + G - 1) // G + 1)]
 
-    def perfile_encode():
+    def perfile(kind):
+        gpu.sync()
+        gpu.record(evs[0])
+        t0 = time.time()
         for k in range(nfs):
             b0, b1 = int(firsts[k]), int(firsts[k + 1])
             if b1 > b0:
-                gpu.encode_blocks(d_in.ptr, d_tab.addr(b0 * 32), b1 - b0, bl, level, d_pslab.addr(b0 * B),
-                                  d_plen.addr(b0 * 4))
+                if kind == "encode":
+                    # (the plain path writes block b's payload at out + (b << block_log): a file's blocks from its slab 0)
+                    gpu.encode_blocks(d_in.ptr, d_tab.addr(b0 * 32), b1 - b0, bl, level, d_pslab.ptr, d_plen.addr(b0 * 4))
+                else:
+                    gpu.decode_blocks(d_comp.ptr, d_dtab.addr(b0 * 32), b1 - b0, bl, d_out.ptr, d_err.addr(b0 * 4))
+            if k % G == G - 1 or k == nfs - 1:
+                gpu.record(evs[k // G + 1])
+        gpu.sync()
+        host_s = time.time() - t0
+        groups = np.array([gpu.elapsed_ms(evs[0], evs[i + 1]) for i in range((nfs + G - 1) // G)], dtype=np.float64)
+        done = groups[np.arange(nfs) // G]
+        return float(done.max()), done, host_s
 
-    def perfile_decode():
-        for k in range(nfs):
-            b0, b1 = int(firsts[k]), int(firsts[k + 1])
-            if b1 > b0:
-                gpu.decode_blocks(d_comp.ptr, d_dtab.addr(b0 * 32), b1 - b0, bl, d_out.ptr, d_err.addr(b0 * 4))
-
-    perfile_encode()
-    pf_enc_ms = timed(perfile_encode, 1)
-    plen = d_plen.download(nbs * 4, dtype=np.uint32)
-    same = bool(np.array_equal(plen, lens[:nbs]))
-    perfile_decode()
-    pf_dec_ms = timed(perfile_decode, 1)
-    # the batched form on the same sample
-    with pyzero.Plan(gpu, src[:nbs], rl[:nbs], bl, a.streams) as sp:
-        d_s2 = gpu.alloc(max(1, sp.slab_bytes))
-        sp.tune(d_in.ptr, level, d_s2.ptr, d_len.ptr)
-        sp.use_graphs(True)
-        sp.encode(d_in.ptr, level, d_s2.ptr, d_len.ptr)
-        sb_enc_ms = timed(lambda: sp.encode(d_in.ptr, level, d_s2.ptr, d_len.ptr), a.steps)
-        sp.set_stored(coff[:nbs], clen[:nbs], np.where(stored[:nbs], 0, 1))
-        sp.decode(d_comp.ptr, d_out.ptr, d_err.ptr)
-        sb_dec_ms = timed(lambda: sp.decode(d_comp.ptr, d_out.ptr, d_err.ptr), a.steps)
-    log(f"sample of {nfs} files ({sample_bytes / 2**20:.1f} MiB): per-file encode {sample_bytes / pf_enc_ms / 1e6:.2f} "
-        f"GB/s, decode {sample_bytes / pf_dec_ms / 1e6:.2f} GB/s; batched encode {sample_bytes / sb_enc_ms / 1e6:.2f} "
-        f"GB/s, decode {sample_bytes / sb_dec_ms / 1e6:.2f} GB/s; per-file payloads identical: {same}")
+    pf_enc_ms, pf_enc_done, pf_enc_host = perfile("encode")
+    plen = d_plen.download(nb * 4, dtype=np.uint32)
+    same = bool(np.array_equal(plen, lens))
+    gpu._check(gpu.lib.zgpu_fill(gpu.h, d_out.ptr, 0, total, None))  # (the per-file decode must write it all again)
+    pf_dec_ms, pf_dec_done, pf_dec_host = perfile("decode")
+    pf_ok = not d_err.download(dtype=np.uint32).any() and np.array_equal(d_out.download(), data)
+    # the batched form: one plan call over the backlog, every file done when it ends
+    def pct(v, q):
+        return round(float(np.percentile(v, q)), 3)
+    lat = {"definition": "a backlog of every file submitted at t = 0; latency = time until the file's results are on "
+                         "the device (per-file form: event after every 64th file's call; batched form: the plan "
+                         "call's end, for every file)",
+           "per_file_compress_ms": {"p50": pct(pf_enc_done, 50), "p99": pct(pf_enc_done, 99)},
+           "per_file_decompress_ms": {"p50": pct(pf_dec_done, 50), "p99": pct(pf_dec_done, 99)},
+           "batched_compress_ms": {"p50": round(enc_ms, 3), "p99": round(enc_ms, 3)},
+           "batched_decompress_ms": {"p50": round(dec_ms, 3), "p99": round(dec_ms, 3)}}
+    log(f"per-file form over all {nfs} files: encode {total / pf_enc_ms / 1e6:.2f} GB/s (host {pf_enc_host:.1f} s), "
+        f"decode {total / pf_dec_ms / 1e6:.2f} GB/s, round trip {'identical' if pf_ok else 'MISMATCH'}, block sizes "
+        f"identical to the batched form: {same}; latency {lat}")
     line = {
-        "metric": "batched uneven-file compress/decompress GB/s vs the plain per-file form",
+        "metric": "batched uneven-file compress/decompress GB/s and p50/p99 latency vs the plain per-file form",
         "value": round(total / (dec_ms * 1e-3) / 1e9, 3), "unit": "GB/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(dec_ms, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u8", "data": "synthetic (seeded generators; no dataset)",
         "config": {"workload": f"{a.files} files, lognormal sizes (median 8 KiB, sigma 2, 64 B..64 MiB), seed-1 LZ "
                                f"bytes, level {level}, 2^{bl} blocks, one plan; step = decode of every file",
-                   "files": a.files, "input_bytes": total, "blocks": nb, "streams": a.streams},
+                   "files": a.files, "input_bytes": total, "blocks": nb},
         "batched": {"compress_GBps": round(total / (enc_ms * 1e-3) / 1e9, 3),
                     "decompress_GBps": round(total / (dec_ms * 1e-3) / 1e9, 3), "ratio": round(comp.size / total, 5),
-                    "roundtrip_identical": ok, "classes": plan.classes()},
-        "per_file_sample": {"files": nfs, "bytes": sample_bytes,
-                            "per_file_compress_GBps": round(sample_bytes / (pf_enc_ms * 1e-3) / 1e9, 3),
-                            "per_file_decompress_GBps": round(sample_bytes / (pf_dec_ms * 1e-3) / 1e9, 3),
-                            "batched_compress_GBps": round(sample_bytes / (sb_enc_ms * 1e-3) / 1e9, 3),
-                            "batched_decompress_GBps": round(sample_bytes / (sb_dec_ms * 1e-3) / 1e9, 3),
-                            "payloads_identical": same},
+                    "roundtrip_identical": ok, "classes": plan.classes(), "passes": plan.passes,
+                    "stage_workgroups_per_cu": plan.stage_occupancy()},
+        "per_file": {"files": nfs, "bytes": total,
+                     "compress_GBps": round(total / (pf_enc_ms * 1e-3) / 1e9, 3),
+                     "decompress_GBps": round(total / (pf_dec_ms * 1e-3) / 1e9, 3),
+                     "roundtrip_identical": pf_ok, "block_sizes_identical_to_batched": same},
+        "latency": lat,
     }
     plan.close()
     print(json.dumps(line), flush=True)
@@ -338,8 +377,6 @@ def main():
     ap.add_argument("--seconds", type=float, default=1.0, help="serve workload: arrival window")
     ap.add_argument("--slots", type=int, default=6, help="serve workload: batches in flight")
     ap.add_argument("--files", type=int, default=200000, help="uneven workload: number of files")
-    ap.add_argument("--sample-files", type=int, default=2000, help="uneven workload: files timed per-file")
-    ap.add_argument("--streams", type=int, default=4, help="uneven workload: streams of the plan")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (blocks would shard across GPUs with one "
@@ -461,6 +498,19 @@ def main():
     roof["frac"] = round(roof["achieved"] * 1e9 / HBM_PEAK, 4)
     roof["traffic"] = traffic_of(f"k_dec_{dom}", n, level, bl)
     roof["whole_decode_frac"] = round(alg / (dec_ms * 1e-3) / HBM_PEAK, 4)
+    # the encoder: the whole pass over the same algorithmic bytes (raw read, payloads written), and its dominant
+    # kernel, the match finder, whose own algorithmic bytes per position are the sorted entry read (4), the byte
+    # (1) and the match written (4); stage times from one 1 GiB launch
+    enc = dict(enc_stages)
+    gib_bytes = min(nb, batch) * B
+    enc_roof = {"bound": "hbm", "achieved": round(alg / (enc_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s"}
+    enc_roof["frac"] = round(enc_roof["achieved"] * 1e9 / HBM_PEAK, 5)
+    if "match" in enc:
+        enc_roof["k_enc_match"] = {"algorithmic_bytes": 9 * gib_bytes, "ms": round(enc["match"], 3),
+                                   "achieved": round(9 * gib_bytes / (enc["match"] * 1e-3) / 1e9, 1)}
+        enc_roof["k_enc_match"]["frac"] = round(enc_roof["k_enc_match"]["achieved"] * 1e9 / HBM_PEAK, 5)
+        enc_roof["k_enc_match"]["traffic"] = traffic_of("k_enc_match", n, level, bl)
 
     # ---- random 1 GiB: every block stored
     for buf in (d_comp, d_out):
@@ -530,6 +580,7 @@ def main():
         "copy_GBps": round(2 * n / (copy_ms * 1e-3) / 1e9, 1),
         "encode_stage_ms_per_GiB": {k: round(v, 3) for k, v in enc_stages},
         "roofline": roof,
+        "encode_roofline": enc_roof,
         "cpu_baseline": cpu,
         "device": props["name"] or f"{props['arch']}, {props['cu_count']} CUs",  # (the name is empty on some boxes)
     }

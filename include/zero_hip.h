@@ -157,11 +157,13 @@ int zgpu_crc32_ranges(zgpu_t* g, int which, uint32_t n, const uint64_t* off, con
 /* ---- the batched form: the blocks of many files, of uneven lengths, planned together (no reference counterpart:
  * the reference launches its stages per 16 MiB batch of full blocks, src/gpu_codec.h:153-229, and assumes every
  * block but the last is full, :160). A block of r bytes runs in a slab of 2^c bytes, c = max(10, ceil(log2 r)) up
- * to block_log; each size class runs the plain path's kernels on its own slabs, classes spread over `streams`
- * streams, and every payload is byte-identical to the plain path's (the hash keys keep block_log's width). */
+ * to block_log (its size class). Every stage of the encoder and decoder is one persistent launch over the blocks of
+ * every class, whose workgroups take work items from per-XCD device queues, largest class first; blocks run in
+ * passes of at most 2 GiB of slabs (one scratch arena, reused). Every payload is byte-identical to the plain
+ * path's (the hash keys keep block_log's width). */
 typedef struct zgpu_plan zgpu_plan_t;
 /* nb blocks: block b's raw bytes are raw_len[b] (<= 2^block_log) bytes at offset src[b] of the input buffer (host
- * arrays, copied). streams 1..8. */
+ * arrays, copied). streams 1..8 (kept for callers: the persistent stages spread over the whole GPU on one). */
 int zgpu_plan_create(zgpu_t* g, uint32_t nb, const uint64_t* src, const uint32_t* raw_len, uint32_t block_log,
                      int streams, zgpu_plan_t** out);
 void zgpu_plan_destroy(zgpu_plan_t* p);
@@ -171,6 +173,14 @@ int zgpu_plan_slab_offsets(const zgpu_plan_t* p, uint64_t* slab_off);
 /* the size classes: per class its slab log, block count, bucket bits and parse walkers; returns the class count */
 int zgpu_plan_classes(const zgpu_plan_t* p, uint32_t* block_log, uint32_t* blocks, uint32_t* hash_bits,
                       uint32_t* walkers, int cap);
+/* the workgroups per compute unit of each persistent stage (hash, place, sort_s, sort_m, sort_l, match, parse,
+ * tokens, rans, pack, decode streams, decode replay); returns the stage count */
+int zgpu_plan_stage_occupancy(zgpu_plan_t* p, uint32_t* occ, int cap);
+/* the slab bytes a pass holds at most (2 GiB unless set; at least 1 MiB): the encoder's scratch arena is about 19
+ * times that */
+int zgpu_plan_pass_bytes(zgpu_plan_t* p, uint64_t slab_bytes);
+/* the passes the plan's blocks run in (-1 before the first encode or decode) */
+int zgpu_plan_passes(const zgpu_plan_t* p);
 /* encodes every block (on the context's stream; asynchronous): payload at d_slabs + slab_off[b], its size (or
  * ZGPU_STORED) at d_out_len[b] */
 int zgpu_plan_encode(zgpu_plan_t* p, const uint8_t* d_in, int level, uint8_t* d_slabs, uint32_t* d_out_len);
@@ -180,8 +190,9 @@ int zgpu_plan_set_stored(zgpu_plan_t* p, const uint64_t* comp_off, const uint32_
 /* decodes every block: raw bytes to d_out + src[b] (the input layout of zgpu_plan_create), err[b] as
  * zgpu_decode_blocks */
 int zgpu_plan_decode(zgpu_plan_t* p, const uint8_t* d_comp, uint8_t* d_out, uint32_t* d_err);
-/* times each class's output-invariant launch parameters (bucket bits, parse walkers) on this input and keeps the
- * fastest (synchronous; writes d_slabs / d_out_len as zgpu_plan_encode does) */
+/* chooses the output-invariant parameters on this input and keeps the fastest: each class's bucket bits (the
+ * hash-bucket depth) and parse walkers (the parse-group width), and each encoder stage's workgroups per compute
+ * unit (synchronous; writes d_slabs / d_out_len as zgpu_plan_encode does) */
 int zgpu_plan_tune(zgpu_plan_t* p, const uint8_t* d_in, int level, uint8_t* d_slabs, uint32_t* d_out_len);
 /* on: a call's whole launch sequence is captured into a hipGraph the first time and replayed when the same buffers
  * and level come again */

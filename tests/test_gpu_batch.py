@@ -30,9 +30,11 @@ def files(seed):
     return out
 
 
-def run_plan(gpu, data, sizes, bl, level, streams=4, tune=False, graphs=0):
+def run_plan(gpu, data, sizes, bl, level, streams=4, tune=False, graphs=0, pass_bytes=None):
     src, rl, fi = pyzero.file_blocks(sizes, bl)
     with pyzero.Plan(gpu, src, rl, bl, streams) as plan:
+        if pass_bytes:
+            plan.set_pass_bytes(pass_bytes)
         d_in = gpu.alloc(max(1, data.size))
         d_in.upload(data)
         d_slabs = gpu.alloc(max(1, plan.slab_bytes))
@@ -51,6 +53,9 @@ def run_plan(gpu, data, sizes, bl, level, streams=4, tune=False, graphs=0):
                             (1, slabs[int(off[b]): int(off[b]) + int(lens[b])].tobytes()) for b in range(plan.nb)])
             d_slabs.upload(np.zeros(max(1, plan.slab_bytes), dtype=np.uint8))  # (a replay must write it all again)
         classes = plan.classes()
+        run_plan.passes = plan.passes
+        run_plan.slab_bytes = plan.slab_bytes
+        run_plan.occupancy = plan.stage_occupancy()
         # decode through the plan: payloads back to back
         blocks = results[-1]
         comp = [p if m else data[int(s): int(s) + int(r)].tobytes() for (m, p), s, r in zip(blocks, src, rl)]
@@ -91,14 +96,24 @@ def test_plan_equals_plain_path(gpu, bl, level):
 
 
 def test_plan_tuned_and_graph_replays(gpu):
+    """the persistent stages' output does not depend on the tuned parameters (bucket bits, parse walkers, workgroups
+    per compute unit), on graph replays, or on how the blocks are cut into passes"""
     fs = files(7)
     sizes = [f.size for f in fs]
     data = np.concatenate(fs)
     base, _, _, _ = run_plan(gpu, data, sizes, 18, 5, streams=1)
+    assert run_plan.passes == 1
     tuned, dec, classes, _ = run_plan(gpu, data, sizes, 18, 5, streams=3, tune=True, graphs=3)
     assert len(tuned) == 3 and len(dec) == 3
+    assert set(run_plan.occupancy) >= {"hash", "match", "parse", "dlz"}
     for r in tuned:
         assert r == base[0]
+    for d in dec:
+        assert np.array_equal(d, data)
+    # many passes: a class split over passes, passes of several classes, graph replays of them
+    many, dec, _, _ = run_plan(gpu, data, sizes, 18, 5, graphs=2, pass_bytes=1 << 20)
+    assert run_plan.passes >= max(2, run_plan.slab_bytes >> 20), (run_plan.passes, run_plan.slab_bytes)
+    assert many[0] == base[0] and many[1] == base[0]
     for d in dec:
         assert np.array_equal(d, data)
 

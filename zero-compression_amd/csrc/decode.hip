@@ -163,15 +163,12 @@ __device__ inline void store512(uint8_t* d, const uint8_t* stg) {
 }
 
 // ---------------------------------------------------------------- the streams: one wavefront per (block, stream)
-__global__ __launch_bounds__(64) void k_dec_streams(const Block* __restrict__ blocks, const uint8_t* __restrict__ comp,
-                                                    uint32_t blog, uint8_t* __restrict__ lit, uint8_t* __restrict__ tok,
-                                                    uint8_t* __restrict__ ext, uint8_t* __restrict__ sym,
-                                                    uint8_t* __restrict__ grp, uint32_t* __restrict__ err) {
+// stream s of block b
+__device__ inline void dec_stream(const Block* __restrict__ blocks, const uint8_t* __restrict__ comp, uint32_t blog,
+                                  uint8_t* __restrict__ lit, uint8_t* __restrict__ tok, uint8_t* __restrict__ ext,
+                                  uint8_t* __restrict__ sym, uint8_t* __restrict__ grp, uint32_t* __restrict__ err,
+                                  uint32_t b, uint32_t s) {
   const Geo G = geo_of(blog);
-  // the literal streams of every block first, then the other four streams: a literal stream is usually many times
-  // longer than the others, and started first its wavefronts do not trail the kernel's end
-  const uint32_t nb = gridDim.x / 5u;
-  const uint32_t b = blockIdx.x % nb, s = blockIdx.x / nb;
   const uint32_t lane = threadIdx.x;
   const Block blk = blocks[b];
   if (blk.method != 1u) return;
@@ -317,6 +314,15 @@ __global__ __launch_bounds__(64) void k_dec_streams(const Block* __restrict__ bl
     }
   }
   if (ballot(pos > lnl) && lane == 0) atomicOr(&err[b], E_LANE);
+}
+// the literal streams of every block first, then the other four streams: a literal stream is usually many times
+// longer than the others, and started first its wavefronts do not trail the kernel's end
+__global__ __launch_bounds__(64) void k_dec_streams(const Block* __restrict__ blocks, const uint8_t* __restrict__ comp,
+                                                    uint32_t blog, uint8_t* __restrict__ lit, uint8_t* __restrict__ tok,
+                                                    uint8_t* __restrict__ ext, uint8_t* __restrict__ sym,
+                                                    uint8_t* __restrict__ grp, uint32_t* __restrict__ err) {
+  const uint32_t nb = gridDim.x / 5u;
+  dec_stream(blocks, comp, blog, lit, tok, ext, sym, grp, err, blockIdx.x % nb, blockIdx.x / nb);
 }
 
 // ---------------------------------------------------------------- the LZ replay: one wavefront per block
@@ -646,11 +652,11 @@ __device__ inline void ring_flush(const uint8_t* ring, uint8_t* o, uint32_t a, u
 // requested; then the chunk's groups are written one after another (a match reads output from before its group
 // only, so the tokens of one group copy side by side). Checks follow the sequential decoder
 // (src/codec2.h:173-202), plus the group rule.
-__global__ __launch_bounds__(64) void k_dec_lz(const Block* __restrict__ blocks, const uint8_t* __restrict__ comp,
-                                               uint32_t blog, const uint8_t* __restrict__ lit,
-                                               const uint8_t* __restrict__ tok, const uint8_t* __restrict__ ext,
-                                               const uint8_t* __restrict__ sym, const uint8_t* __restrict__ grp,
-                                               uint8_t* out, uint32_t* __restrict__ err) {
+__device__ inline void dec_lz_block(const Block* __restrict__ blocks, const uint8_t* __restrict__ comp, uint32_t blog,
+                                    const uint8_t* __restrict__ lit, const uint8_t* __restrict__ tok,
+                                    const uint8_t* __restrict__ ext, const uint8_t* __restrict__ sym,
+                                    const uint8_t* __restrict__ grp, uint8_t* out, uint32_t* __restrict__ err,
+                                    uint32_t b) {
   __shared__ uint32_t sflag[64];  // token k of the chunk starts a group
   __shared__ uint32_t sgs[64];    // output start of the chunk's k-th group
   __shared__ uint32_t sgk[64];    // first token (in the chunk) of the chunk's k-th group
@@ -658,7 +664,6 @@ __global__ __launch_bounds__(64) void k_dec_lz(const Block* __restrict__ blocks,
   __shared__ uint32_t sown[64];
   __shared__ uint8_t ring[RING + 16];  // the block's most recent output
   const Geo G = geo_of(blog);
-  const uint32_t b = blockIdx.x;
   const uint32_t lane = threadIdx.x;
   const Block blk = blocks[b];
   uint8_t* o = out + blk.dst;
@@ -853,6 +858,13 @@ __global__ __launch_bounds__(64) void k_dec_lz(const Block* __restrict__ blocks,
   wave_fence();
   ring_flush(ring, o, fl_a, fl_e);
   wave_copy(o + op, litb + li, nl - li);
+}
+__global__ __launch_bounds__(64) void k_dec_lz(const Block* __restrict__ blocks, const uint8_t* __restrict__ comp,
+                                               uint32_t blog, const uint8_t* __restrict__ lit,
+                                               const uint8_t* __restrict__ tok, const uint8_t* __restrict__ ext,
+                                               const uint8_t* __restrict__ sym, const uint8_t* __restrict__ grp,
+                                               uint8_t* out, uint32_t* __restrict__ err) {
+  dec_lz_block(blocks, comp, blog, lit, tok, ext, sym, grp, out, err, blockIdx.x);
 }
 
 }  // namespace zk

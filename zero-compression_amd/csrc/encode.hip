@@ -94,11 +94,11 @@ __host__ __device__ inline uint32_t pbits_of(uint32_t blog, uint32_t hbits) {
   const uint32_t want = blog > 13u ? blog - 13u : 0u;
   return want < hbits ? want : hbits;
 }
-__global__ __launch_bounds__(1024) void k_enc_hash(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
-                                                   uint32_t blog, uint32_t hbits, uint32_t pbits,
-                                                   uint32_t* __restrict__ tmp, uint32_t* __restrict__ bstart) {
+__device__ inline void enc_hash_block(const Block* __restrict__ blocks, const uint8_t* __restrict__ in, uint32_t blog,
+                                      uint32_t hbits, uint32_t pbits, uint32_t* __restrict__ tmp,
+                                      uint32_t* __restrict__ bstart, uint32_t b) {
   extern __shared__ uint32_t cnt[];  // HB counters, 16 wavefront sums, the partition cursors, the claims
-  const uint32_t b = blockIdx.x, tid = threadIdx.x;
+  const uint32_t tid = threadIdx.x;
   const uint32_t HBITS = hbits, HB = 1u << HBITS;
   const Block blk = blocks[b];
   const uint8_t* inb = in + blk.src;
@@ -177,16 +177,19 @@ __global__ __launch_bounds__(1024) void k_enc_hash(const Block* __restrict__ blo
     x = nx;
   }
 }
+__global__ __launch_bounds__(1024) void k_enc_hash(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
+                                                   uint32_t blog, uint32_t hbits, uint32_t pbits,
+                                                   uint32_t* __restrict__ tmp, uint32_t* __restrict__ bstart) {
+  enc_hash_block(blocks, in, blog, hbits, pbits, tmp, bstart, blockIdx.x);
+}
 
 // one workgroup per (block, partition): the partition's positions (from tmp) to their buckets
-__global__ __launch_bounds__(1024) void k_enc_place(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
-                                                    uint32_t nb, uint32_t blog, uint32_t hbits, uint32_t keybits,
-                                                    uint32_t pbits, const uint32_t* __restrict__ tmp,
-                                                    const uint32_t* __restrict__ bstart, uint32_t* __restrict__ srt) {
+__device__ inline void enc_place_part(const Block* __restrict__ blocks, const uint8_t* __restrict__ in, uint32_t blog,
+                                      uint32_t hbits, uint32_t keybits, uint32_t pbits,
+                                      const uint32_t* __restrict__ tmp, const uint32_t* __restrict__ bstart,
+                                      uint32_t* __restrict__ srt, uint32_t b, uint32_t P) {
   extern __shared__ uint32_t sm[];  // bucket cursors (2^(hbits - pbits)), then the partition's entries
-  const uint32_t wg = xcd_swizzle(blockIdx.x, gridDim.x);
-  const uint32_t b = wg >> pbits, P = wg & ((1u << pbits) - 1u), tid = threadIdx.x;
-  if (b >= nb) return;
+  const uint32_t tid = threadIdx.x;
   const uint32_t HB = 1u << hbits, pb = hbits - pbits, NB = 1u << pb;
   const Block blk = blocks[b];
   const uint8_t* inb = in + blk.src;
@@ -221,6 +224,15 @@ __global__ __launch_bounds__(1024) void k_enc_place(const Block* __restrict__ bl
   __syncthreads();
   if (in_lds)
     for (uint32_t i = tid; i < m; i += 1024) sb[i] = buf[i];
+}
+__global__ __launch_bounds__(1024) void k_enc_place(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
+                                                    uint32_t nb, uint32_t blog, uint32_t hbits, uint32_t keybits,
+                                                    uint32_t pbits, const uint32_t* __restrict__ tmp,
+                                                    const uint32_t* __restrict__ bstart, uint32_t* __restrict__ srt) {
+  const uint32_t wg = xcd_swizzle(blockIdx.x, gridDim.x);
+  const uint32_t b = wg >> pbits, P = wg & ((1u << pbits) - 1u);
+  if (b >= nb) return;  // (uniform over the workgroup)
+  enc_place_part(blocks, in, blog, hbits, keybits, pbits, tmp, bstart, srt, b, P);
 }
 
 // ---------------------------------------------------------------- 2. bucket sorts (the values are distinct)
@@ -304,28 +316,37 @@ __device__ inline void sort_small(uint32_t* base, uint32_t at, uint32_t lo, uint
   }
 }
 
-__global__ __launch_bounds__(256) void k_enc_sort_s(const Block* __restrict__ blocks, uint32_t nb, uint32_t blog,
-                                                    uint32_t hbits, uint32_t* __restrict__ srt,
-                                                    const uint32_t* __restrict__ bstart, SortQueues q) {
+// bucket h of block b on this thread (valid false: a thread with no bucket, which still takes part in the
+// wavefront's sorts of the others)
+__device__ inline void enc_sort_bucket(uint32_t blog, uint32_t hbits, uint32_t* __restrict__ srt,
+                                       const uint32_t* __restrict__ bstart, SortQueues q, uint32_t b, uint32_t h,
+                                       bool valid) {
   const uint32_t HB = 1u << hbits;
-  const uint64_t gid = (uint64_t)xcd_swizzle(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-  const uint32_t b = (uint32_t)(gid / HB), h = (uint32_t)(gid % HB);
   uint32_t lo = 0, k = 0;
-  if (b < nb) {
+  if (valid) {
     const uint32_t* bs = bstart + (size_t)b * (HB + 1);
     lo = bs[h];
     k = bs[h + 1] - lo;
   }
   sort_small(srt + ((size_t)b << blog), 0, lo, k, b, h, q);
 }
+__global__ __launch_bounds__(256) void k_enc_sort_s(const Block* __restrict__ blocks, uint32_t nb, uint32_t blog,
+                                                    uint32_t hbits, uint32_t* __restrict__ srt,
+                                                    const uint32_t* __restrict__ bstart, SortQueues q) {
+  const uint32_t HB = 1u << hbits;
+  const uint64_t gid = (uint64_t)xcd_swizzle(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  const uint32_t b = (uint32_t)(gid / HB), h = (uint32_t)(gid % HB);
+  enc_sort_bucket(blog, hbits, srt, bstart, q, b, h, b < nb);
+}
 
 // queued buckets of 65..4096 entries: one workgroup per bucket, bitonic sort in LDS (padded to a power of two)
-__global__ __launch_bounds__(256) void k_enc_sort_m(uint32_t blog, uint32_t hbits, uint32_t* __restrict__ srt,
-                                                    const uint32_t* __restrict__ bstart, SortQueues q) {
+// the queued buckets w0, w0 + step, ... of q.mid (a workgroup's share of them)
+__device__ inline void enc_sort_m_range(uint32_t blog, uint32_t hbits, uint32_t* __restrict__ srt,
+                                        const uint32_t* __restrict__ bstart, SortQueues q, uint32_t w0, uint32_t step) {
   __shared__ uint32_t a[4096];
   const uint32_t HB = 1u << hbits;
   const uint32_t total = min(q.counts[0], q.cap);
-  for (uint32_t w = blockIdx.x; w < total; w += gridDim.x) {
+  for (uint32_t w = w0; w < total; w += step) {
     const uint32_t b = q.mid[2 * w], h = q.mid[2 * w + 1];
     const uint32_t* bs = bstart + (size_t)b * (HB + 1);
     const uint32_t lo = bs[h], k = bs[h + 1] - lo;
@@ -350,18 +371,23 @@ __global__ __launch_bounds__(256) void k_enc_sort_m(uint32_t blog, uint32_t hbit
     __syncthreads();
   }
 }
+__global__ __launch_bounds__(256) void k_enc_sort_m(uint32_t blog, uint32_t hbits, uint32_t* __restrict__ srt,
+                                                    const uint32_t* __restrict__ bstart, SortQueues q) {
+  enc_sort_m_range(blog, hbits, srt, bstart, q, blockIdx.x, gridDim.x);
+}
 
 // queued buckets of more than 4096 entries: one workgroup of 1024 per bucket, a stable LSD radix sort with 8-bit
 // digits through a temporary of the same size (tmp, indexed like srt), in-wavefront ranks from ballots.
-__global__ __launch_bounds__(1024) void k_enc_sort_l(uint32_t blog, uint32_t hbits, uint32_t* __restrict__ srt,
-                                                     uint32_t* __restrict__ tmp, const uint32_t* __restrict__ bstart,
-                                                     SortQueues q) {
+// the queued buckets w0, w0 + step, ... of q.large
+__device__ inline void enc_sort_l_range(uint32_t blog, uint32_t hbits, uint32_t* __restrict__ srt,
+                                        uint32_t* __restrict__ tmp, const uint32_t* __restrict__ bstart, SortQueues q,
+                                        uint32_t w0, uint32_t step) {
   __shared__ uint32_t base[256], tile_tot[256];
   __shared__ uint32_t wcnt[16][256];
   const uint32_t HB = 1u << hbits;
   const uint32_t tid = threadIdx.x, wv = tid >> 6;
   const uint32_t total = min(q.counts[1], q.cap);
-  for (uint32_t w = blockIdx.x; w < total; w += gridDim.x) {
+  for (uint32_t w = w0; w < total; w += step) {
     const uint32_t b = q.large[2 * w], h = q.large[2 * w + 1];
     const uint32_t* bs = bstart + (size_t)b * (HB + 1);
     const uint32_t lo = bs[h], k = bs[h + 1] - lo;
@@ -413,6 +439,11 @@ __global__ __launch_bounds__(1024) void k_enc_sort_l(uint32_t blog, uint32_t hbi
       }
     }
   }
+}
+__global__ __launch_bounds__(1024) void k_enc_sort_l(uint32_t blog, uint32_t hbits, uint32_t* __restrict__ srt,
+                                                     uint32_t* __restrict__ tmp, const uint32_t* __restrict__ bstart,
+                                                     SortQueues q) {
+  enc_sort_l_range(blog, hbits, srt, tmp, bstart, q, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------- 3. match finder (one thread per sorted slot)
@@ -533,13 +564,13 @@ __device__ inline void match_slots(const uint8_t* __restrict__ inb, uint32_t n, 
   else if (i >= count && i < n) mb[i] = 0;  // (the last 3 positions are never in srt: exactly these slots)
 }
 
-__global__ __launch_bounds__(256) void k_enc_match(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
-                                                   uint32_t blog, uint32_t keybits, uint32_t ncand, uint32_t lencap,
-                                                   const uint32_t* __restrict__ srt, uint32_t* __restrict__ match) {
+// 256 sorted slots of block b, from slot s0 (a wavefront takes 64 of them)
+__device__ inline void enc_match_slots(const Block* __restrict__ blocks, const uint8_t* __restrict__ in, uint32_t blog,
+                                       uint32_t keybits, uint32_t ncand, uint32_t lencap,
+                                       const uint32_t* __restrict__ srt, uint32_t* __restrict__ match, uint32_t b,
+                                       uint32_t s0) {
   __shared__ uint32_t sh[4][(MX + 64) * 10];
-  const uint32_t wg = xcd_swizzle(blockIdx.x, gridDim.x);
-  const uint32_t b = wg >> (blog - 8);
-  const uint32_t i0 = ((wg << 8) + (threadIdx.x & ~63u)) & ((1u << blog) - 1u);
+  const uint32_t i0 = s0 + (threadIdx.x & ~63u);
   const Block blk = blocks[b];
   const uint32_t n = enc_len(blk, blog);
   if (i0 >= n) return;  // (uniform over the wavefront)
@@ -552,6 +583,13 @@ __global__ __launch_bounds__(256) void k_enc_match(const Block* __restrict__ blo
     match_slots<4>(inb, n, count, blog, keybits, min(ncand, MX), lencap, sb, i0, mb, sh[threadIdx.x >> 6]);
   else
     match_slots<8>(inb, n, count, blog, keybits, min(ncand, MX), min(lencap, 32u), sb, i0, mb, sh[threadIdx.x >> 6]);
+}
+__global__ __launch_bounds__(256) void k_enc_match(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
+                                                   uint32_t blog, uint32_t keybits, uint32_t ncand, uint32_t lencap,
+                                                   const uint32_t* __restrict__ srt, uint32_t* __restrict__ match) {
+  const uint32_t wg = xcd_swizzle(blockIdx.x, gridDim.x);
+  enc_match_slots(blocks, in, blog, keybits, ncand, lencap, srt, match, wg >> (blog - 8),
+                  (wg << 8) & ((1u << blog) - 1u));
 }
 
 // ---------------------------------------------------------------- 4. lazy matching (kernels/zero.cl:239-255)
@@ -636,12 +674,12 @@ __device__ inline uint32_t join_walk(uint32_t* m, const uint8_t* inb, uint32_t n
 // place: where the chain meets a walk the two coincide from there on (the chain is a pure function of the position),
 // so only the walk's prefix is wrong; where it never meets one, the walk is wrong whole.
 // mark[b] receives the bitmap of the chain (B bits).
-__global__ __launch_bounds__(1024) void k_enc_parse(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
-                                                    uint32_t blog, uint32_t walkers, uint32_t lencap,
-                                                    uint32_t* __restrict__ match, uint32_t* __restrict__ mark) {
+__device__ inline void enc_parse_block(const Block* __restrict__ blocks, const uint8_t* __restrict__ in, uint32_t blog,
+                                       uint32_t walkers, uint32_t lencap, uint32_t* __restrict__ match,
+                                       uint32_t* __restrict__ mark, uint32_t b) {
   extern __shared__ uint32_t sm[];  // bitmap (B/32 words), ex, sin, sout (walkers words each), windows (16 each)
   const uint32_t B = 1u << blog, SEG = B / walkers, T = blockDim.x;  // (T >= walkers)
-  const uint32_t b = blockIdx.x, k = threadIdx.x;
+  const uint32_t k = threadIdx.x;
   uint32_t* bm = sm;
   uint32_t* ex = sm + B / 32u;
   uint32_t* sin = ex + walkers;    // a segment's entry as walked
@@ -730,6 +768,11 @@ __global__ __launch_bounds__(1024) void k_enc_parse(const Block* __restrict__ bl
   uint32_t* mk = mark + ((size_t)b << (blog - 5));
   for (uint32_t i = k; i < B / 32u; i += T) mk[i] = bm[i];
 }
+__global__ __launch_bounds__(1024) void k_enc_parse(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
+                                                    uint32_t blog, uint32_t walkers, uint32_t lencap,
+                                                    uint32_t* __restrict__ match, uint32_t* __restrict__ mark) {
+  enc_parse_block(blocks, in, blog, walkers, lencap, match, mark, blockIdx.x);
+}
 
 // ---------------------------------------------------------------- 6. tokens and streams (one wavefront per block)
 // From the chain: lit (the literal bytes), then per token (in order) its literal run L, length M, distance D, and
@@ -750,13 +793,12 @@ struct TokBufs {
   uint32_t* cnt;     // 8 per block
 };
 
-__global__ __launch_bounds__(64) void k_enc_tokens(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
-                                                   uint32_t blog, uint32_t lencap,
-                                                   const uint32_t* __restrict__ match,
-                                                   const uint32_t* __restrict__ mark, TokBufs T) {
+__device__ inline void enc_tokens_block(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
+                                        uint32_t blog, uint32_t lencap, const uint32_t* __restrict__ match,
+                                        const uint32_t* __restrict__ mark, TokBufs T, uint32_t b) {
   __shared__ uint32_t stage[48];
   const Geo G = geo_of(blog);
-  const uint32_t b = blockIdx.x, lane = threadIdx.x;
+  const uint32_t lane = threadIdx.x;
   const Block blk = blocks[b];
   const uint8_t* inb = in + blk.src;
   const uint32_t n = enc_len(blk, blog);
@@ -902,6 +944,12 @@ __global__ __launch_bounds__(64) void k_enc_tokens(const Block* __restrict__ blo
     cn[0] = nt; cn[1] = nl; cn[2] = ei; cn[3] = (bp + 7u) >> 3; cn[4] = ng;
   }
 }
+__global__ __launch_bounds__(64) void k_enc_tokens(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
+                                                   uint32_t blog, uint32_t lencap,
+                                                   const uint32_t* __restrict__ match,
+                                                   const uint32_t* __restrict__ mark, TokBufs T) {
+  enc_tokens_block(blocks, in, blog, lencap, match, mark, T, blockIdx.x);
+}
 
 // ---------------------------------------------------------------- 7. rANS per (block, stream)
 constexpr uint32_t RING_STRIDE = 144;  // a lane's LDS ring of 128 bytes, 8 of spill, padded to 16
@@ -943,15 +991,13 @@ __device__ inline uint32_t stream_count(const uint32_t* cn, uint32_t s) {
   return s == 0 ? cn[1] : s == 2 ? cn[2] : s == 4 ? cn[4] : cn[0];
 }
 
-__global__ __launch_bounds__(64) void k_enc_rans(uint32_t blog, TokBufs T, RansBufs R) {
+// stream s of block b (g = 5 b + s indexes the outputs)
+__device__ inline void enc_rans_stream(uint32_t blog, TokBufs T, RansBufs R, uint32_t b, uint32_t s) {
   __shared__ uint32_t hist[256];
   __shared__ uint32_t fc[256];   // freq | cum << 16
   __shared__ uint32_t rcp[256];  // ceil(2^(31+s) / freq), s = ceil(log2 freq)
   const Geo G = geo_of(blog);
-  // the literal streams of every block first, then the other four streams (a literal stream is usually many times
-  // longer: started first, its wavefronts do not trail the kernel's end); g = 5 b + s indexes the outputs
-  const uint32_t nbk = gridDim.x / 5u;
-  const uint32_t b = blockIdx.x % nbk, s = blockIdx.x / nbk, g = 5u * b + s, lane = threadIdx.x;
+  const uint32_t g = 5u * b + s, lane = threadIdx.x;
   const uint32_t* cn = T.cnt + (size_t)b * 8;
   const uint32_t n = stream_count(cn, s);
   const uint8_t* sy = stream_syms(T, G, b, s);
@@ -1142,6 +1188,12 @@ __global__ __launch_bounds__(64) void k_enc_rans(uint32_t blog, TokBufs T, RansB
     si[0] = kind; si[1] = kind ? total : n; si[2] = npres; si[3] = n;
   }
 }
+// the literal streams of every block first, then the other four streams (a literal stream is usually many times
+// longer: started first, its wavefronts do not trail the kernel's end)
+__global__ __launch_bounds__(64) void k_enc_rans(uint32_t blog, TokBufs T, RansBufs R) {
+  const uint32_t nbk = gridDim.x / 5u;
+  enc_rans_stream(blog, T, R, blockIdx.x % nbk, blockIdx.x / nbk);
+}
 
 // ---------------------------------------------------------------- 8. the payload (one workgroup per block)
 // out + b*B receives the ZB2 payload (src/codec2.h:140-151) and outlen[b] its size, or 0xFFFFFFFF when the payload
@@ -1157,11 +1209,11 @@ __device__ inline void wg_copy(uint8_t* d, const uint8_t* s, uint32_t n) {
   if (full + threadIdx.x < n) d[full + threadIdx.x] = s[full + threadIdx.x];
 }
 
-__global__ __launch_bounds__(256) void k_enc_pack(const Block* __restrict__ blocks, uint32_t blog, TokBufs T,
-                                                  RansBufs R, uint8_t* __restrict__ out, uint32_t* __restrict__ outlen) {
+__device__ inline void enc_pack_block(const Block* __restrict__ blocks, uint32_t blog, TokBufs T, RansBufs R,
+                                      uint8_t* __restrict__ out, uint32_t* __restrict__ outlen, uint32_t b) {
   __shared__ uint32_t loff[65];
   const Geo G = geo_of(blog);
-  const uint32_t b = blockIdx.x, tid = threadIdx.x;
+  const uint32_t tid = threadIdx.x;
   if (blocks[b].raw_len > G.B) { if (tid == 0) outlen[b] = BAD_BLOCK; return; }
   const uint32_t n = blocks[b].raw_len;
   const uint32_t* cn = T.cnt + (size_t)b * 8;
@@ -1216,6 +1268,10 @@ __global__ __launch_bounds__(256) void k_enc_pack(const Block* __restrict__ bloc
   }
   const uint8_t* ob = (const uint8_t*)(T.offw + (size_t)b * (G.OCAP / 4u + 1u));
   wg_copy(o + off[5], ob, nob);
+}
+__global__ __launch_bounds__(256) void k_enc_pack(const Block* __restrict__ blocks, uint32_t blog, TokBufs T,
+                                                  RansBufs R, uint8_t* __restrict__ out, uint32_t* __restrict__ outlen) {
+  enc_pack_block(blocks, blog, T, R, out, outlen, blockIdx.x);
 }
 
 }  // namespace zk

@@ -39,6 +39,7 @@ EXPORTS = (
     "zgpu_set_profile", "zgpu_stage_times",
     "zgpu_plan_create", "zgpu_plan_destroy", "zgpu_plan_slab_bytes", "zgpu_plan_slab_offsets", "zgpu_plan_classes",
     "zgpu_plan_encode", "zgpu_plan_set_stored", "zgpu_plan_decode", "zgpu_plan_tune", "zgpu_plan_use_graphs",
+    "zgpu_plan_stage_occupancy", "zgpu_plan_passes", "zgpu_plan_pass_bytes",
     "zgpu_serve_bench",
 )
 
@@ -99,6 +100,8 @@ def load_library(path=LIB_PATH):
         "zgpu_plan_encode": (i32, [vp, vp, i32, vp, vp]), "zgpu_plan_set_stored": (i32, [vp, vp, vp, vp]),
         "zgpu_plan_decode": (i32, [vp, vp, vp, vp]), "zgpu_plan_tune": (i32, [vp, vp, i32, vp, vp]),
         "zgpu_plan_use_graphs": (i32, [vp, i32]),
+        "zgpu_plan_stage_occupancy": (i32, [vp, vp, i32]), "zgpu_plan_passes": (i32, [vp]),
+        "zgpu_plan_pass_bytes": (i32, [vp, u64]),
         "zgpu_serve_bench": (i32, [vp, ctypes.POINTER(ServeCfg), vp, ctypes.c_uint64, vp, ctypes.c_uint64,
                                    ctypes.POINTER(ServeReport)]),
         "zgpu_stream_create": (i32, [vp, ctypes.POINTER(vp)]), "zgpu_stream_destroy": (i32, [vp, vp]),
@@ -441,6 +444,23 @@ class Plan:
 
     def use_graphs(self, on=True):
         self.gpu._check(self.lib.zgpu_plan_use_graphs(self.h, 1 if on else 0))
+
+    STAGES = ("hash", "place", "sort_s", "sort_m", "sort_l", "match", "parse", "tokens", "rans", "pack",
+              "dstreams", "dlz")
+
+    def stage_occupancy(self):
+        """workgroups per compute unit of each persistent stage"""
+        occ = np.zeros(16, dtype=np.uint32)
+        n = self.lib.zgpu_plan_stage_occupancy(self.h, _ptr(occ), 16)
+        return dict(zip(self.STAGES, [int(x) for x in occ[:n]]))
+
+    @property
+    def passes(self):
+        return int(self.lib.zgpu_plan_passes(self.h))
+
+    def set_pass_bytes(self, n):
+        """at most n slab bytes per pass (the encoder's scratch is about 19 n)"""
+        self.gpu._check(self.lib.zgpu_plan_pass_bytes(self.h, int(n)))
 
 
 def file_blocks(sizes, block_log):
