@@ -176,9 +176,15 @@ int zgpu_plan_classes(const zgpu_plan_t* p, uint32_t* block_log, uint32_t* block
 /* the workgroups per compute unit of each persistent stage (hash, place, sort_s, sort_m, sort_l, match, parse,
  * tokens, rans, pack, decode streams, decode replay); returns the stage count */
 int zgpu_plan_stage_occupancy(zgpu_plan_t* p, uint32_t* occ, int cap);
+/* sets them (n stages from the first; 1..64 each) */
+int zgpu_plan_set_stage_occupancy(zgpu_plan_t* p, const uint32_t* occ, int n);
 /* the slab bytes a pass holds at most (2 GiB unless set; at least 1 MiB): the encoder's scratch arena is about 19
  * times that */
 int zgpu_plan_pass_bytes(zgpu_plan_t* p, uint64_t slab_bytes);
+/* on: the next encode or decode records an event after every stage of every pass (graphs are not used meanwhile);
+ * zgpu_plan_stage_ms then gives each stage's time summed over the passes (ms[stage], the stage order above) */
+int zgpu_plan_profile(zgpu_plan_t* p, int on);
+int zgpu_plan_stage_ms(zgpu_plan_t* p, float* ms, int cap);
 /* the passes the plan's blocks run in (-1 before the first encode or decode) */
 int zgpu_plan_passes(const zgpu_plan_t* p);
 /* encodes every block (on the context's stream; asynchronous): payload at d_slabs + slab_off[b], its size (or

@@ -37,6 +37,8 @@ constexpr const char* PSTAGE_NAME[P_STAGES] = {"hash",   "place",  "sort_s", "so
                                                "parse",  "tokens", "rans",   "pack",   "dstreams", "dlz"};
 constexpr uint32_t MAX_SEGS = 16;            // classes (or parts of one) in a pass
 constexpr uint64_t PASS_SLAB = 1ull << 31;   // slab bytes of a pass (by default)
+constexpr uint32_t QSTRIDE = 32;             // u32s from one queue head to the next: a 128-byte line each
+constexpr size_t CTR_WORDS = P_STAGES * 8 * QSTRIDE + 4 * MAX_SEGS;
 
 // one class's blocks in a pass, as the persistent kernels see them (device memory)
 struct PSeg {
@@ -96,8 +98,9 @@ __device__ inline uint32_t take(uint32_t* q, uint32_t n) {
 #define PLAN_LOOP(stage, PER, G)                                                                      \
   const uint32_t x = blockIdx.x & 7u;                                                                 \
   uint32_t c, bl, sub;                                                                                \
-  for (uint32_t j0 = take(ctr + (stage) * 8u + x, (G)), j = j0; locate(t, x, j, PER, c, bl, sub);     \
-       j = j + 1u - j0 < (G) ? j + 1u : (j0 = take(ctr + (stage) * 8u + x, (G))))
+  uint32_t* const head = ctr + ((stage) * 8u + x) * QSTRIDE;                                          \
+  for (uint32_t j0 = take(head, (G)), j = j0; locate(t, x, j, PER, c, bl, sub);                       \
+       j = j + 1u - j0 < (G) ? j + 1u : (j0 = take(head, (G))))
 
 __global__ __launch_bounds__(1024) void k_plan_hash(const PTab* __restrict__ t, uint32_t* ctr, const Block* enc,
                                                     const uint8_t* in) {
@@ -135,7 +138,9 @@ __global__ __launch_bounds__(1024) void k_plan_sort_l(const PTab* __restrict__ t
 }
 __global__ __launch_bounds__(256) void k_plan_match(const PTab* __restrict__ t, uint32_t* ctr, const Block* enc,
                                                     const uint8_t* in, uint32_t ncand, uint32_t lencap) {
-  PLAN_LOOP(P_MATCH, [](const PSeg& g) { return 1u << (g.blog - 8u); }, 16u) {
+  // one item a claim: the items in flight on an XCD then cover a fraction of one block, whose scattered match writes
+  // stay in that XCD's L2 until whole lines are written (16 a claim spread them over several blocks: twice the time)
+  PLAN_LOOP(P_MATCH, [](const PSeg& g) { return 1u << (g.blog - 8u); }, 1u) {
     const PSeg& g = t->s[c];
     enc_match_slots(enc + g.first, in, g.blog, g.keybits, ncand, lencap, g.srt, g.mt, bl, sub << 8);
   }
@@ -157,7 +162,8 @@ __global__ __launch_bounds__(64) void k_plan_tokens(const PTab* __restrict__ t, 
 __global__ __launch_bounds__(64) void k_plan_rans(const PTab* __restrict__ t, uint32_t* ctr) {
   const uint32_t x = blockIdx.x & 7u;
   uint32_t c, bl, s;
-  for (uint32_t j = take(ctr + P_RANS * 8u + x, 1u); locate5(t, x, j, c, bl, s); j = take(ctr + P_RANS * 8u + x, 1u)) {
+  uint32_t* const head = ctr + (P_RANS * 8u + x) * QSTRIDE;
+  for (uint32_t j = take(head, 1u); locate5(t, x, j, c, bl, s); j = take(head, 1u)) {
     const PSeg& g = t->s[c];
     enc_rans_stream(g.blog, g.T, g.R, bl, s);
   }
@@ -173,8 +179,8 @@ __global__ __launch_bounds__(64) void k_plan_dstreams(const PTab* __restrict__ t
                                                       const uint8_t* comp, uint32_t* res) {
   const uint32_t x = blockIdx.x & 7u;
   uint32_t c, bl, s;
-  for (uint32_t j = take(ctr + P_DSTREAMS * 8u + x, 1u); locate5(t, x, j, c, bl, s);
-       j = take(ctr + P_DSTREAMS * 8u + x, 1u)) {
+  uint32_t* const head = ctr + (P_DSTREAMS * 8u + x) * QSTRIDE;
+  for (uint32_t j = take(head, 1u); locate5(t, x, j, c, bl, s); j = take(head, 1u)) {
     const PSeg& g = t->s[c];
     dec_stream(dec + g.first, comp, g.blog, g.dlit, g.dtok, g.dext, g.dsym, g.dgrp, res + g.first, bl, s);
   }
@@ -250,12 +256,14 @@ struct zgpu_plan {
   Block* d_dec = nullptr;  // plan order (after zgpu_plan_set_stored)
   uint32_t* d_order = nullptr;
   uint32_t* d_tmp = nullptr;
-  uint32_t* d_ctr = nullptr;  // queue heads [P_STAGES][8], then the sort queues' counts [MAX_SEGS][4]
+  uint32_t* d_ctr = nullptr;  // queue heads [P_STAGES][8] (QSTRIDE apart), then the sort queues' counts [MAX_SEGS][4]
   uint32_t occ[P_STAGES];
   uint64_t pass_slab = PASS_SLAB;
   bool tables_ready = false, use_graphs = false;
   std::map<std::tuple<int, const void*, const void*, const void*, int>, hipGraphExec_t> graphs;
-  std::vector<hipEvent_t> stage_ev;  // (the tuner's)
+  std::vector<hipEvent_t> stage_ev;  // (the tuner's, and the profile's)
+  bool profile = false;
+  int profiled_op = -1;  // the call the events belong to
 };
 
 static void plan_drop_graphs(zgpu_plan* p) {
@@ -334,7 +342,7 @@ static int plan_tables(zgpu_plan* p) {
       q.mt = (uint32_t*)(base + L.mtch);
       q.bstart = (uint32_t*)(base + L.bstart);
       q.mark = (uint32_t*)(base + L.mark);
-      q.q = SortQueues{(uint32_t*)(base + L.qmid), (uint32_t*)(base + L.qlarge), p->d_ctr + P_STAGES * 8 + 4 * k, L.qcap};
+      q.q = SortQueues{(uint32_t*)(base + L.qmid), (uint32_t*)(base + L.qlarge), p->d_ctr + P_STAGES * 8 * QSTRIDE + 4 * k, L.qcap};
       q.T = TokBufs{base + L.lit,  (uint32_t*)(base + L.LB), (uint32_t*)(base + L.TM), (uint32_t*)(base + L.TD),
                     base + L.tok,  base + L.sym,              base + L.grp,              base + L.ext,
                     (uint32_t*)(base + L.offw), (uint32_t*)(base + L.cnt)};
@@ -381,7 +389,7 @@ static int plan_launch(zgpu_plan* p, int op, const uint8_t* d_src, uint8_t* d_ds
   hipStream_t st = g->stream;
   uint32_t ncand, lencap;
   level_params(op == 0 ? (uint32_t)level : 5u, ncand, lencap);
-  const size_t nctr = (P_STAGES * 8 + 4 * MAX_SEGS) * 4;
+  const size_t nctr = CTR_WORDS * 4;
   if (op == 1) k_fill<<<(unsigned)std::min<size_t>((p->nb * 4 + 255) / 256, 1024), 256, 0, st>>>((uint8_t*)p->d_tmp, 0, (size_t)p->nb * 4);
   int e = 0;
   auto mark = [&]() { if (ev) (void)hipEventRecord(ev[e++], st); };
@@ -424,10 +432,26 @@ static int plan_launch(zgpu_plan* p, int op, const uint8_t* d_src, uint8_t* d_ds
   return ZGPU_OK;
 }
 
+static int plan_events(zgpu_plan* p) {  // one before the first stage, one after each stage of every pass
+  const size_t nev = 1 + p->passes.size() * 10;
+  while (p->stage_ev.size() < nev) {
+    hipEvent_t e;
+    HIPCHK(p->g, hipEventCreate(&e));
+    p->stage_ev.push_back(e);
+  }
+  return ZGPU_OK;
+}
+
 static int plan_call(zgpu_plan* p, int op, const uint8_t* d_src, uint8_t* d_dst, uint32_t* d_res, int level) {
   if (p->nb == 0) return ZGPU_OK;
   if (!p->tables_ready)
     if (int rc = plan_tables(p)) return rc;
+  if (p->profile) {  // (events between the stages: no graph)
+    if (int rc = plan_events(p)) return rc;
+    HIPCHK(p->g, hipEventRecord(p->stage_ev[0], p->g->stream));
+    p->profiled_op = op;
+    return plan_launch(p, op, d_src, d_dst, d_res, level, p->stage_ev.data() + 1);
+  }
   if (!p->use_graphs) return plan_launch(p, op, d_src, d_dst, d_res, level, nullptr);
   const auto key = std::make_tuple(op, (const void*)d_src, (const void*)d_dst, (const void*)d_res, level);
   auto it = p->graphs.find(key);
@@ -489,7 +513,7 @@ int zgpu_plan_create(zgpu_t* g, uint32_t nb, const uint64_t* src, const uint32_t
   auto chk = [&](hipError_t e, const char* what) {
     if (e != hipSuccess && rc == ZGPU_OK) rc = hip_fail(g, e, what);
   };
-  chk(hipMalloc((void**)&p->d_ctr, (P_STAGES * 8 + 4 * MAX_SEGS) * 4), "hipMalloc");
+  chk(hipMalloc((void**)&p->d_ctr, CTR_WORDS * 4), "hipMalloc");
   if (nb) {
     std::vector<Block>& tab = p->htab;
     tab.resize(nb);
@@ -542,12 +566,40 @@ int zgpu_plan_stage_occupancy(zgpu_plan_t* p, uint32_t* occ, int cap) {
   return (int)P_STAGES;
 }
 
+int zgpu_plan_set_stage_occupancy(zgpu_plan_t* p, const uint32_t* occ, int n) {
+  for (int k = 0; k < n && k < (int)P_STAGES; ++k)
+    if (occ[k] < 1 || occ[k] > 64) return fail(p->g, ZGPU_ERR_ARG, "1..64 workgroups per compute unit");
+  for (int k = 0; k < n && k < (int)P_STAGES; ++k) p->occ[k] = occ[k];
+  plan_drop_graphs(p);
+  return ZGPU_OK;
+}
+
 int zgpu_plan_pass_bytes(zgpu_plan_t* p, uint64_t slab_bytes) {
   if (slab_bytes < (1u << 20)) return fail(p->g, ZGPU_ERR_ARG, "a pass holds at least 1 MiB of slabs");
   p->pass_slab = slab_bytes;
   p->tables_ready = false;  // (the passes are laid out again at the next call)
   plan_drop_graphs(p);
   return ZGPU_OK;
+}
+
+int zgpu_plan_profile(zgpu_plan_t* p, int on) {
+  p->profile = on != 0;
+  return ZGPU_OK;
+}
+
+int zgpu_plan_stage_ms(zgpu_plan_t* p, float* ms, int cap) {
+  if (p->profiled_op < 0) return fail(p->g, ZGPU_ERR_ARG, "no profiled call (zgpu_plan_profile)");
+  const uint32_t first = p->profiled_op == 0 ? 0u : (uint32_t)P_DSTREAMS, nst = p->profiled_op == 0 ? 10u : 2u;
+  for (int k = 0; k < cap; ++k) ms[k] = 0;
+  for (size_t ps = 0; ps < p->passes.size(); ++ps)
+    for (uint32_t s = 0; s < nst; ++s) {
+      float t = 0;
+      const size_t e = ps * nst + s;
+      HIPCHK(p->g, hipEventSynchronize(p->stage_ev[e + 1]));
+      HIPCHK(p->g, hipEventElapsedTime(&t, p->stage_ev[e], p->stage_ev[e + 1]));
+      if ((int)(first + s) < cap) ms[first + s] += t;
+    }
+  return (int)P_STAGES;
 }
 
 int zgpu_plan_passes(const zgpu_plan_t* p) {

@@ -39,7 +39,8 @@ EXPORTS = (
     "zgpu_set_profile", "zgpu_stage_times",
     "zgpu_plan_create", "zgpu_plan_destroy", "zgpu_plan_slab_bytes", "zgpu_plan_slab_offsets", "zgpu_plan_classes",
     "zgpu_plan_encode", "zgpu_plan_set_stored", "zgpu_plan_decode", "zgpu_plan_tune", "zgpu_plan_use_graphs",
-    "zgpu_plan_stage_occupancy", "zgpu_plan_passes", "zgpu_plan_pass_bytes",
+    "zgpu_plan_stage_occupancy", "zgpu_plan_set_stage_occupancy", "zgpu_plan_passes", "zgpu_plan_pass_bytes",
+    "zgpu_plan_profile", "zgpu_plan_stage_ms",
     "zgpu_serve_bench",
 )
 
@@ -101,7 +102,8 @@ def load_library(path=LIB_PATH):
         "zgpu_plan_decode": (i32, [vp, vp, vp, vp]), "zgpu_plan_tune": (i32, [vp, vp, i32, vp, vp]),
         "zgpu_plan_use_graphs": (i32, [vp, i32]),
         "zgpu_plan_stage_occupancy": (i32, [vp, vp, i32]), "zgpu_plan_passes": (i32, [vp]),
-        "zgpu_plan_pass_bytes": (i32, [vp, u64]),
+        "zgpu_plan_pass_bytes": (i32, [vp, u64]), "zgpu_plan_set_stage_occupancy": (i32, [vp, vp, i32]),
+        "zgpu_plan_profile": (i32, [vp, i32]), "zgpu_plan_stage_ms": (i32, [vp, vp, i32]),
         "zgpu_serve_bench": (i32, [vp, ctypes.POINTER(ServeCfg), vp, ctypes.c_uint64, vp, ctypes.c_uint64,
                                    ctypes.POINTER(ServeReport)]),
         "zgpu_stream_create": (i32, [vp, ctypes.POINTER(vp)]), "zgpu_stream_destroy": (i32, [vp, vp]),
@@ -453,6 +455,24 @@ class Plan:
         occ = np.zeros(16, dtype=np.uint32)
         n = self.lib.zgpu_plan_stage_occupancy(self.h, _ptr(occ), 16)
         return dict(zip(self.STAGES, [int(x) for x in occ[:n]]))
+
+    def profile(self, on=True):
+        self.gpu._check(self.lib.zgpu_plan_profile(self.h, 1 if on else 0))
+
+    def stage_ms(self):
+        """each stage's time in the last profiled call, summed over the passes"""
+        ms = np.zeros(16, dtype=np.float32)
+        n = self.lib.zgpu_plan_stage_ms(self.h, _ptr(ms), 16)
+        if n < 0:
+            self.gpu._check(n)
+        return {k: round(float(v), 3) for k, v in zip(self.STAGES, ms[:n]) if v > 0}
+
+    def set_stage_occupancy(self, **occ):
+        """workgroups per compute unit of the named stages"""
+        cur = self.stage_occupancy()
+        cur.update(occ)
+        a = np.array([cur[k] for k in self.STAGES], dtype=np.uint32)
+        self.gpu._check(self.lib.zgpu_plan_set_stage_occupancy(self.h, _ptr(a), a.size))
 
     @property
     def passes(self):
