@@ -7,6 +7,10 @@
 //
 // Safety: every read stays inside its block's compressed bytes and every write inside its block's output and
 // scratch, whatever the data; damage sets err[b] bits and never faults.
+//
+// The bodies dec_stream and dec_lz_block are __forceinline__: the plain kernels and the persistent ones (plan.hip)
+// share them, and a shared body compiled as a call would take the call convention's registers (k_dec_lz: 260 VGPRs
+// instead of 96).
 #include "zcommon.h"
 
 namespace zk {
@@ -164,7 +168,7 @@ __device__ inline void store512(uint8_t* d, const uint8_t* stg) {
 
 // ---------------------------------------------------------------- the streams: one wavefront per (block, stream)
 // stream s of block b
-__device__ inline void dec_stream(const Block* __restrict__ blocks, const uint8_t* __restrict__ comp, uint32_t blog,
+__device__ __forceinline__ void dec_stream(const Block* __restrict__ blocks, const uint8_t* __restrict__ comp, uint32_t blog,
                                   uint8_t* __restrict__ lit, uint8_t* __restrict__ tok, uint8_t* __restrict__ ext,
                                   uint8_t* __restrict__ sym, uint8_t* __restrict__ grp, uint32_t* __restrict__ err,
                                   uint32_t b, uint32_t s) {
@@ -652,7 +656,7 @@ __device__ inline void ring_flush(const uint8_t* ring, uint8_t* o, uint32_t a, u
 // requested; then the chunk's groups are written one after another (a match reads output from before its group
 // only, so the tokens of one group copy side by side). Checks follow the sequential decoder
 // (src/codec2.h:173-202), plus the group rule.
-__device__ inline void dec_lz_block(const Block* __restrict__ blocks, const uint8_t* __restrict__ comp, uint32_t blog,
+__device__ __forceinline__ void dec_lz_block(const Block* __restrict__ blocks, const uint8_t* __restrict__ comp, uint32_t blog,
                                     const uint8_t* __restrict__ lit, const uint8_t* __restrict__ tok,
                                     const uint8_t* __restrict__ ext, const uint8_t* __restrict__ sym,
                                     const uint8_t* __restrict__ grp, uint8_t* out, uint32_t* __restrict__ err,

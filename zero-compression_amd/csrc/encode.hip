@@ -14,6 +14,10 @@
 //   k_enc_tokens    the chain -> literals and tokens; groups, recent offsets, tok/ext/sym/groups, offset bits
 //   k_enc_rans      per (block, stream): histogram, 12-bit frequencies, 64 lanes of rANS
 //   k_enc_pack      the payload, or "stored" when it is not smaller than the block
+//
+// Each stage's body is a __forceinline__ function of (block, item), which the plain kernels here and the persistent
+// ones in plan.hip both call: a body that two kernels share is otherwise compiled as a call, and the call's register
+// convention more than doubles a kernel's VGPRs (so halves its occupancy or worse).
 #include "zcommon.h"
 
 namespace zk {
@@ -94,7 +98,7 @@ __host__ __device__ inline uint32_t pbits_of(uint32_t blog, uint32_t hbits) {
   const uint32_t want = blog > 13u ? blog - 13u : 0u;
   return want < hbits ? want : hbits;
 }
-__device__ inline void enc_hash_block(const Block* __restrict__ blocks, const uint8_t* __restrict__ in, uint32_t blog,
+__device__ __forceinline__ void enc_hash_block(const Block* __restrict__ blocks, const uint8_t* __restrict__ in, uint32_t blog,
                                       uint32_t hbits, uint32_t pbits, uint32_t* __restrict__ tmp,
                                       uint32_t* __restrict__ bstart, uint32_t b) {
   extern __shared__ uint32_t cnt[];  // HB counters, 16 wavefront sums, the partition cursors, the claims
@@ -184,7 +188,7 @@ __global__ __launch_bounds__(1024) void k_enc_hash(const Block* __restrict__ blo
 }
 
 // one workgroup per (block, partition): the partition's positions (from tmp) to their buckets
-__device__ inline void enc_place_part(const Block* __restrict__ blocks, const uint8_t* __restrict__ in, uint32_t blog,
+__device__ __forceinline__ void enc_place_part(const Block* __restrict__ blocks, const uint8_t* __restrict__ in, uint32_t blog,
                                       uint32_t hbits, uint32_t keybits, uint32_t pbits,
                                       const uint32_t* __restrict__ tmp, const uint32_t* __restrict__ bstart,
                                       uint32_t* __restrict__ srt, uint32_t b, uint32_t P) {
@@ -318,7 +322,7 @@ __device__ inline void sort_small(uint32_t* base, uint32_t at, uint32_t lo, uint
 
 // bucket h of block b on this thread (valid false: a thread with no bucket, which still takes part in the
 // wavefront's sorts of the others)
-__device__ inline void enc_sort_bucket(uint32_t blog, uint32_t hbits, uint32_t* __restrict__ srt,
+__device__ __forceinline__ void enc_sort_bucket(uint32_t blog, uint32_t hbits, uint32_t* __restrict__ srt,
                                        const uint32_t* __restrict__ bstart, SortQueues q, uint32_t b, uint32_t h,
                                        bool valid) {
   const uint32_t HB = 1u << hbits;
@@ -341,7 +345,7 @@ __global__ __launch_bounds__(256) void k_enc_sort_s(const Block* __restrict__ bl
 
 // queued buckets of 65..4096 entries: one workgroup per bucket, bitonic sort in LDS (padded to a power of two)
 // the queued buckets w0, w0 + step, ... of q.mid (a workgroup's share of them)
-__device__ inline void enc_sort_m_range(uint32_t blog, uint32_t hbits, uint32_t* __restrict__ srt,
+__device__ __forceinline__ void enc_sort_m_range(uint32_t blog, uint32_t hbits, uint32_t* __restrict__ srt,
                                         const uint32_t* __restrict__ bstart, SortQueues q, uint32_t w0, uint32_t step) {
   __shared__ uint32_t a[4096];
   const uint32_t HB = 1u << hbits;
@@ -379,7 +383,7 @@ __global__ __launch_bounds__(256) void k_enc_sort_m(uint32_t blog, uint32_t hbit
 // queued buckets of more than 4096 entries: one workgroup of 1024 per bucket, a stable LSD radix sort with 8-bit
 // digits through a temporary of the same size (tmp, indexed like srt), in-wavefront ranks from ballots.
 // the queued buckets w0, w0 + step, ... of q.large
-__device__ inline void enc_sort_l_range(uint32_t blog, uint32_t hbits, uint32_t* __restrict__ srt,
+__device__ __forceinline__ void enc_sort_l_range(uint32_t blog, uint32_t hbits, uint32_t* __restrict__ srt,
                                         uint32_t* __restrict__ tmp, const uint32_t* __restrict__ bstart, SortQueues q,
                                         uint32_t w0, uint32_t step) {
   __shared__ uint32_t base[256], tile_tot[256];
@@ -565,7 +569,7 @@ __device__ inline void match_slots(const uint8_t* __restrict__ inb, uint32_t n, 
 }
 
 // 256 sorted slots of block b, from slot s0 (a wavefront takes 64 of them)
-__device__ inline void enc_match_slots(const Block* __restrict__ blocks, const uint8_t* __restrict__ in, uint32_t blog,
+__device__ __forceinline__ void enc_match_slots(const Block* __restrict__ blocks, const uint8_t* __restrict__ in, uint32_t blog,
                                        uint32_t keybits, uint32_t ncand, uint32_t lencap,
                                        const uint32_t* __restrict__ srt, uint32_t* __restrict__ match, uint32_t b,
                                        uint32_t s0) {
@@ -674,7 +678,7 @@ __device__ inline uint32_t join_walk(uint32_t* m, const uint8_t* inb, uint32_t n
 // place: where the chain meets a walk the two coincide from there on (the chain is a pure function of the position),
 // so only the walk's prefix is wrong; where it never meets one, the walk is wrong whole.
 // mark[b] receives the bitmap of the chain (B bits).
-__device__ inline void enc_parse_block(const Block* __restrict__ blocks, const uint8_t* __restrict__ in, uint32_t blog,
+__device__ __forceinline__ void enc_parse_block(const Block* __restrict__ blocks, const uint8_t* __restrict__ in, uint32_t blog,
                                        uint32_t walkers, uint32_t lencap, uint32_t* __restrict__ match,
                                        uint32_t* __restrict__ mark, uint32_t b) {
   extern __shared__ uint32_t sm[];  // bitmap (B/32 words), ex, sin, sout (walkers words each), windows (16 each)
@@ -793,7 +797,7 @@ struct TokBufs {
   uint32_t* cnt;     // 8 per block
 };
 
-__device__ inline void enc_tokens_block(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
+__device__ __forceinline__ void enc_tokens_block(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
                                         uint32_t blog, uint32_t lencap, const uint32_t* __restrict__ match,
                                         const uint32_t* __restrict__ mark, TokBufs T, uint32_t b) {
   __shared__ uint32_t stage[48];
@@ -992,7 +996,7 @@ __device__ inline uint32_t stream_count(const uint32_t* cn, uint32_t s) {
 }
 
 // stream s of block b (g = 5 b + s indexes the outputs)
-__device__ inline void enc_rans_stream(uint32_t blog, TokBufs T, RansBufs R, uint32_t b, uint32_t s) {
+__device__ __forceinline__ void enc_rans_stream(uint32_t blog, TokBufs T, RansBufs R, uint32_t b, uint32_t s) {
   __shared__ uint32_t hist[256];
   __shared__ uint32_t fc[256];   // freq | cum << 16
   __shared__ uint32_t rcp[256];  // ceil(2^(31+s) / freq), s = ceil(log2 freq)
@@ -1209,7 +1213,7 @@ __device__ inline void wg_copy(uint8_t* d, const uint8_t* s, uint32_t n) {
   if (full + threadIdx.x < n) d[full + threadIdx.x] = s[full + threadIdx.x];
 }
 
-__device__ inline void enc_pack_block(const Block* __restrict__ blocks, uint32_t blog, TokBufs T, RansBufs R,
+__device__ __forceinline__ void enc_pack_block(const Block* __restrict__ blocks, uint32_t blog, TokBufs T, RansBufs R,
                                       uint8_t* __restrict__ out, uint32_t* __restrict__ outlen, uint32_t b) {
   __shared__ uint32_t loff[65];
   const Geo G = geo_of(blog);
