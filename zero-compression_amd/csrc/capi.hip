@@ -76,6 +76,7 @@ struct zgpu {
   uint32_t* xpow = nullptr;  // x^(8 m 256^j) mod P, 3 x 256 (CRC combine)
   DBuf enc, dec, aux;  // encoder and decoder scratch of the device-resident forms; staging of the small host forms
   DBuf rng;            // byte-range CRC tables
+  DBuf stage;          // (pinned host memory) the host-buffer forms' block tables and per-block results
   // the last zgpu_compress input [0] (in hin) and zgpu_decompress output [1] (in hout); length 0: none
   const uint8_t* last_data[2] = {nullptr, nullptr};
   uint64_t last_len[2] = {0, 0};
@@ -103,6 +104,17 @@ static int hip_fail(zgpu_t* g, hipError_t e, const char* what) {
     if (e_ != hipSuccess) return hip_fail(g, e_, #call); \
   } while (0)
 
+// pinned host memory for the copies of a pipelined call: an asynchronous copy to or from pageable memory makes the
+// host wait until the copy is done, so the next chunk would not be queued until this one had finished
+static int ensure_pinned(zgpu_t* g, DBuf& b, size_t bytes) {
+  if (b.n >= bytes) return ZGPU_OK;
+  if (b.p) HIPCHK(g, hipHostFree(b.p));
+  b.p = nullptr;
+  b.n = 0;
+  HIPCHK(g, hipHostMalloc(&b.p, bytes, hipHostMallocDefault));
+  b.n = bytes;
+  return ZGPU_OK;
+}
 static int ensure(zgpu_t* g, DBuf& b, size_t bytes) {
   if (b.n >= bytes) return ZGPU_OK;
   if (b.p) HIPCHK(g, hipFree(b.p));
@@ -301,6 +313,7 @@ void zgpu_close(zgpu_t* g) {
   }
   for (DBuf* b : {&g->enc, &g->dec, &g->aux, &g->rng, &g->hin, &g->hout})
     if (b->p) (void)hipFree(b->p);
+  if (g->stage.p) (void)hipHostFree(g->stage.p);
   for (hipEvent_t e : g->pool) (void)hipEventDestroy(e);
   if (g->xpow) (void)hipFree(g->xpow);
   (void)hipStreamDestroy(g->stream);
@@ -636,7 +649,11 @@ int zgpu_compress(zgpu_t* g, const uint8_t* in, uint64_t n, uint32_t blog, int l
     if (int rc = ensure(g, l.meta, meta)) return rc;
   }
   uint8_t* d_in = (uint8_t*)g->hin.p;
-  std::vector<zgpu_block_t> tab(nb);
+  // pinned: the block table, then the lengths and CRCs as they come back (copied to the caller's arrays at the end)
+  if (int rc = ensure_pinned(g, g->stage, al((size_t)nb * 32) + 2 * al((size_t)nb * 4))) return rc;
+  zgpu_block_t* tab = (zgpu_block_t*)g->stage.p;
+  uint32_t* h_len = (uint32_t*)((uint8_t*)g->stage.p + al((size_t)nb * 32));
+  uint32_t* h_crc = (uint32_t*)((uint8_t*)h_len + al((size_t)nb * 4));
   for (uint32_t b = 0; b < nb; ++b) {
     tab[b] = zgpu_block_t{};
     tab[b].src = (uint64_t)(b % per) * B;  // (within its chunk's part of the input)
@@ -653,7 +670,7 @@ int zgpu_compress(zgpu_t* g, const uint8_t* in, uint64_t n, uint32_t blog, int l
     uint64_t* d_goff = (uint64_t*)((uint8_t*)d_crc + al((size_t)per * 4));
     uint64_t* d_tot = (uint64_t*)((uint8_t*)d_goff + al((size_t)per * 8));
     HIPCHK(g, hipMemcpyAsync(d_in + o, in + o, bytes, hipMemcpyHostToDevice, l.st));
-    HIPCHK(g, hipMemcpyAsync(d_tab, tab.data() + b0, (size_t)k * 32, hipMemcpyHostToDevice, l.st));
+    HIPCHK(g, hipMemcpyAsync(d_tab, tab + b0, (size_t)k * 32, hipMemcpyHostToDevice, l.st));
     if (int rc = encode_run(g, d_in + o, (const Block*)d_tab, k, S, level, (uint8_t*)l.slab.p, d_len,
                             (uint8_t*)l.work.p, l.st, c == 0))
       return rc;
@@ -663,9 +680,9 @@ int zgpu_compress(zgpu_t* g, const uint8_t* in, uint64_t n, uint32_t blog, int l
     if (crc) {
       k_crc32<<<(k + 3) / 4, 256, 0, l.st>>>((const Block*)d_tab, k, d_in + o, 0, g->xpow, d_crc);
       HIPCHK(g, hipGetLastError());
-      HIPCHK(g, hipMemcpyAsync(crc + b0, d_crc, (size_t)k * 4, hipMemcpyDeviceToHost, l.st));
+      HIPCHK(g, hipMemcpyAsync(h_crc + b0, d_crc, (size_t)k * 4, hipMemcpyDeviceToHost, l.st));
     }
-    HIPCHK(g, hipMemcpyAsync(out_len + b0, d_len, (size_t)k * 4, hipMemcpyDeviceToHost, l.st));
+    HIPCHK(g, hipMemcpyAsync(h_len + b0, d_len, (size_t)k * 4, hipMemcpyDeviceToHost, l.st));
     HIPCHK(g, hipMemcpyAsync(l.h_total, d_tot, 8, hipMemcpyDeviceToHost, l.st));
     HIPCHK(g, hipEventRecord(l.ev, l.st));
     return ZGPU_OK;
@@ -685,6 +702,8 @@ int zgpu_compress(zgpu_t* g, const uint8_t* in, uint64_t n, uint32_t blog, int l
       if (int rc = front(c + 2)) return rc;  // (after this lane's download: the stream keeps them in order)
   }
   for (Lane& l : g->lanes) HIPCHK(g, hipStreamSynchronize(l.st));
+  memcpy(out_len, h_len, (size_t)nb * 4);
+  if (crc) memcpy(crc, h_crc, (size_t)nb * 4);
   g->last_data[0] = d_in;
   g->last_len[0] = n;
   for (uint32_t b = 0; b < nb; ++b)
@@ -699,7 +718,6 @@ int zgpu_decompress(zgpu_t* g, const uint8_t* comp, const uint32_t* comp_len, co
   g->last_len[0] = g->last_len[1] = 0;
   if (nb == 0) return ZGPU_OK;
   const uint64_t B = 1ull << blog;
-  std::vector<zgpu_block_t> tab(nb);
   std::vector<uint64_t> co(nb + 1), ro(nb + 1);
   co[0] = ro[0] = 0;
   for (uint32_t b = 0; b < nb; ++b) {
@@ -717,10 +735,16 @@ int zgpu_decompress(zgpu_t* g, const uint8_t* comp, const uint32_t* comp_len, co
   }
   uint8_t* d_comp = (uint8_t*)g->hin.p;
   uint8_t* d_out = (uint8_t*)g->hout.p;
+  // pinned: the block table, then the error bits and CRCs as they come back (copied out at the end)
+  if (int rc = ensure_pinned(g, g->stage, al((size_t)nb * 32) + 2 * al((size_t)nb * 4))) return rc;
+  zgpu_block_t* tab = (zgpu_block_t*)g->stage.p;
+  uint32_t* h_err = (uint32_t*)((uint8_t*)g->stage.p + al((size_t)nb * 32));
+  uint32_t* h_crc = (uint32_t*)((uint8_t*)h_err + al((size_t)nb * 4));
   for (uint32_t c = 0; c < nchunks; ++c) {  // every size is known: the chunks are queued at once
     Lane& l = g->lanes[c & 1];
     const uint32_t b0 = c * per, k = std::min(per, nb - b0);
     for (uint32_t b = b0; b < b0 + k; ++b) {
+      tab[b] = zgpu_block_t{};
       tab[b].src = co[b] - co[b0];
       tab[b].dst = ro[b] - ro[b0];
       tab[b].raw_len = raw_len[b];
@@ -733,19 +757,21 @@ int zgpu_decompress(zgpu_t* g, const uint8_t* comp, const uint32_t* comp_len, co
     uint32_t* d_err = (uint32_t*)(m + al((size_t)per * 32));
     uint32_t* d_crc = (uint32_t*)((uint8_t*)d_err + al((size_t)per * 4));
     HIPCHK(g, hipMemcpyAsync(d_comp + co[b0], comp + co[b0], co[b0 + k] - co[b0], hipMemcpyHostToDevice, l.st));
-    HIPCHK(g, hipMemcpyAsync(d_tab, tab.data() + b0, (size_t)k * 32, hipMemcpyHostToDevice, l.st));
+    HIPCHK(g, hipMemcpyAsync(d_tab, tab + b0, (size_t)k * 32, hipMemcpyHostToDevice, l.st));
     if (int rc = decode_run(g, d_comp + co[b0], (const Block*)d_tab, k, blog, d_out + ro[b0], d_err,
                             (uint8_t*)l.work.p, l.st, c == 0))
       return rc;
     if (crc) {
       k_crc32<<<(k + 3) / 4, 256, 0, l.st>>>((const Block*)d_tab, k, d_out + ro[b0], 1, g->xpow, d_crc);
       HIPCHK(g, hipGetLastError());
-      HIPCHK(g, hipMemcpyAsync(crc + b0, d_crc, (size_t)k * 4, hipMemcpyDeviceToHost, l.st));
+      HIPCHK(g, hipMemcpyAsync(h_crc + b0, d_crc, (size_t)k * 4, hipMemcpyDeviceToHost, l.st));
     }
-    HIPCHK(g, hipMemcpyAsync(err + b0, d_err, (size_t)k * 4, hipMemcpyDeviceToHost, l.st));
+    HIPCHK(g, hipMemcpyAsync(h_err + b0, d_err, (size_t)k * 4, hipMemcpyDeviceToHost, l.st));
     HIPCHK(g, hipMemcpyAsync(out + ro[b0], d_out + ro[b0], ro[b0 + k] - ro[b0], hipMemcpyDeviceToHost, l.st));
   }
   for (Lane& l : g->lanes) HIPCHK(g, hipStreamSynchronize(l.st));
+  memcpy(err, h_err, (size_t)nb * 4);
+  if (crc) memcpy(crc, h_crc, (size_t)nb * 4);
   g->last_data[1] = d_out;
   g->last_len[1] = ro[nb];
   for (uint32_t b = 0; b < nb; ++b)
