@@ -85,22 +85,29 @@ __device__ inline bool locate5(const PTab* __restrict__ t, uint32_t x, uint32_t 
   uint32_t sub;
   return locate(t, x, j % tot, [](const PSeg&) { return 1u; }, c, bl, sub);
 }
-// the next n items of a queue for the whole workgroup (the first of them)
-__device__ inline uint32_t take(uint32_t* q, uint32_t n) {
+// Claims: thread 0 takes the next n items of a queue with one atomic, and the workgroup learns the first of them.
+// A workgroup claims its next items as soon as it starts on the current ones (claim_issue), and reads the answer only
+// when it is done with them (claim_share): the atomic's round trip to memory, a microsecond or more, then overlaps
+// the work instead of preceding every item (small blocks take a few microseconds each).
+__device__ inline uint32_t claim_issue(uint32_t* q, uint32_t n) { return threadIdx.x == 0 ? atomicAdd(q, n) : 0u; }
+__device__ inline uint32_t claim_share(uint32_t v) {  // thread 0's v, for every thread of the workgroup
+  if (blockDim.x <= 64u) return __builtin_amdgcn_readfirstlane(v);  // (one wavefront, every lane active)
   __shared__ uint32_t got;
-  __syncthreads();
-  if (threadIdx.x == 0) got = atomicAdd(q, n);
+  __syncthreads();  // (everyone has read the last one)
+  if (threadIdx.x == 0) got = v;
   __syncthreads();
   return got;
 }
 // the items of this workgroup's XCD queue, claimed G at a time (c, bl, sub: the item's segment, block in it, item in
-// the block); the loop's exit is uniform over the workgroup, as every item is
+// the block); the loop's exit is uniform over the workgroup, as every item is. The claim after the current one is
+// in flight while the current items run (its result is unused when the queue ends first).
 #define PLAN_LOOP(stage, PER, G)                                                                      \
   const uint32_t x = blockIdx.x & 7u;                                                                 \
   uint32_t c, bl, sub;                                                                                \
   uint32_t* const head = ctr + ((stage) * 8u + x) * QSTRIDE;                                          \
-  for (uint32_t j0 = take(head, (G)), j = j0; locate(t, x, j, PER, c, bl, sub);                       \
-       j = j + 1u - j0 < (G) ? j + 1u : (j0 = take(head, (G))))
+  uint32_t j0 = claim_share(claim_issue(head, (G))), ahead = claim_issue(head, (G));                 \
+  for (uint32_t j = j0; locate(t, x, j, PER, c, bl, sub);                                             \
+       j = j + 1u - j0 < (G) ? j + 1u : (j0 = claim_share(ahead), ahead = claim_issue(head, (G)), j0))
 
 __global__ __launch_bounds__(1024) void k_plan_hash(const PTab* __restrict__ t, uint32_t* ctr, const Block* enc,
                                                     const uint8_t* in) {
@@ -163,7 +170,8 @@ __global__ __launch_bounds__(64) void k_plan_rans(const PTab* __restrict__ t, ui
   const uint32_t x = blockIdx.x & 7u;
   uint32_t c, bl, s;
   uint32_t* const head = ctr + (P_RANS * 8u + x) * QSTRIDE;
-  for (uint32_t j = take(head, 1u); locate5(t, x, j, c, bl, s); j = take(head, 1u)) {
+  uint32_t j = claim_share(claim_issue(head, 1u)), ahead = claim_issue(head, 1u);
+  for (; locate5(t, x, j, c, bl, s); j = claim_share(ahead), ahead = claim_issue(head, 1u)) {
     const PSeg& g = t->s[c];
     enc_rans_stream(g.blog, g.T, g.R, bl, s);
   }
@@ -180,12 +188,14 @@ __global__ __launch_bounds__(64) void k_plan_dstreams(const PTab* __restrict__ t
   const uint32_t x = blockIdx.x & 7u;
   uint32_t c, bl, s;
   uint32_t* const head = ctr + (P_DSTREAMS * 8u + x) * QSTRIDE;
-  for (uint32_t j = take(head, 1u); locate5(t, x, j, c, bl, s); j = take(head, 1u)) {
+  uint32_t j = claim_share(claim_issue(head, 1u)), ahead = claim_issue(head, 1u);
+  for (; locate5(t, x, j, c, bl, s); j = claim_share(ahead), ahead = claim_issue(head, 1u)) {
     const PSeg& g = t->s[c];
     dec_stream(dec + g.first, comp, g.blog, g.dlit, g.dtok, g.dext, g.dsym, g.dgrp, res + g.first, bl, s);
   }
 }
-__global__ __launch_bounds__(64) void k_plan_dlz(const PTab* __restrict__ t, uint32_t* ctr, const Block* dec,
+// (at most 96 VGPRs, so that 5 wavefronts fit a SIMD: the persistent loop's state would otherwise take it to 106)
+__global__ __launch_bounds__(64, 5) void k_plan_dlz(const PTab* __restrict__ t, uint32_t* ctr, const Block* dec,
                                                  const uint8_t* comp, uint8_t* out, uint32_t* res) {
   PLAN_LOOP(P_DLZ, [](const PSeg&) { return 1u; }, 1u) {
     const PSeg& g = t->s[c];
@@ -237,8 +247,10 @@ bool shape_fits(const EncShape& S) {  // (within the plain kernels' 160 KiB and 
 }
 
 // workgroups per compute unit of each stage's persistent grid, as the resources allow (LDS, registers): the tuner's
-// starting point
-constexpr uint32_t DEFAULT_OCC[P_STAGES] = {1, 2, 8, 4, 1, 8, 2, 16, 8, 8, 16, 12};
+// starting point for the encoder's stages, and the decoder's setting (the LZ replay is bounded to 96 VGPRs, 5
+// wavefronts per SIMD: 20 one-wavefront workgroups per CU; on 200,000 uneven files it took 22.3 ms at 12 per CU,
+// 18.4 at 16 and 17.6 at 20)
+constexpr uint32_t DEFAULT_OCC[P_STAGES] = {1, 2, 8, 4, 1, 8, 2, 16, 8, 8, 16, 20};
 
 }  // namespace
 
