@@ -152,6 +152,11 @@ __global__ void k_fill(uint8_t* p, uint8_t v, size_t n) {
   for (size_t k = i; k < n; k += stride) p[k] = v;
 }
 
+// zeros over n16 16-byte words from p (16-byte aligned)
+__global__ __launch_bounds__(256) void k_zero16(uint4* __restrict__ p, size_t n16) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) p[i] = uint4{0u, 0u, 0u, 0u};
+}
 // a plain copy, 16 bytes per lane per step (both pointers 16-byte aligned, n a multiple of 16; the host does the rest)
 __global__ __launch_bounds__(256) void k_copy(uint4* __restrict__ d, const uint4* __restrict__ s, size_t n16) {
   const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -444,6 +449,9 @@ static int encode_run(zgpu_t* g, const uint8_t* d_in, const Block* blocks, uint3
   k_enc_sort_m<<<std::min<uint32_t>(2048, L.qcap), 256, 0, st>>>(blog, S.hbits, srt, bstart, q);
   k_enc_sort_l<<<std::min<uint32_t>(1024, L.qcap), 1024, 0, st>>>(blog, S.hbits, srt, mt, bstart, q);
   if (prof) stage(g, st, "sort");
+  // the match array cleared: the match kernel writes only the positions that get a match
+  k_zero16<<<(unsigned)std::min<size_t>(((size_t)nb << blog) / 1024 + 1, 16384), 256, 0, st>>>((uint4*)mt,
+                                                                                              ((size_t)nb << blog) / 4);
   k_enc_match<<<posgroups, 256, 0, st>>>(blocks, d_in, blog, S.keybits, ncand, lencap, srt, mt);
   if (prof) stage(g, st, "match");
   const size_t parse_lds = ((size_t)G.B / 32 + 19 * S.walkers) * 4;

@@ -456,10 +456,11 @@ __global__ __launch_bounds__(1024) void k_enc_sort_l(uint32_t blog, uint32_t hbi
 // tag counts too, as there. Only when one exists, the run check (distance 1) goes first (kernels/zero.cl:209-222),
 // then each candidate in slot order; the longest wins, the first found on a tie (zero.cl's early reject of a
 // candidate whose byte at the best length differs makes its nearer-on-a-tie rule unreachable). A wavefront takes 64
-// consecutive slots of one block. Each lane reads the first W bytes of its position once (W = 16 for a length cap of
-// 16, 32 for one of 32; bytes past the block read as 0) and lanes 0..7 also those of the 8 slots before the
-// wavefront's first; windows, positions and keys go to LDS, and each candidate is compared there, window against
-// window. The key of a candidate is its hash recomputed from its window.
+// consecutive slots of one block. The candidates follow from the sorted entries alone (tag << blog | position, the
+// tag being the key's bits below the bucket's); then each lane that has a candidate or is one reads the first W bytes
+// of its position once (W = 16 for a length cap of 16, 32 for one of 32; bytes past the block read as 0), lanes 0..7
+// also those of the 8 slots before the wavefront's first where they are candidates; windows go to LDS, and each
+// candidate is compared there, window against window.
 constexpr uint32_t MX = 8;             // most candidates of any level
 constexpr uint32_t NOPOS = 0xFFFFFFFFu;  // an LDS entry with no slot behind it
 
@@ -501,47 +502,63 @@ __device__ inline uint32_t common_prefix(const uint32_t (&a)[NW], const uint32_t
 
 template <int NW>
 __device__ inline void match_slots(const uint8_t* __restrict__ inb, uint32_t n, uint32_t count, uint32_t blog,
-                                   uint32_t keybits, uint32_t ncand, uint32_t lencap, const uint32_t* __restrict__ sb,
-                                   uint32_t i0, uint32_t* __restrict__ mb, uint32_t* sh) {
+                                   uint32_t ncand, uint32_t lencap, const uint32_t* __restrict__ sb, uint32_t i0,
+                                   uint32_t* __restrict__ mb, uint32_t* sh) {
   const uint32_t lane = threadIdx.x & 63u, i = i0 + lane;
   const uint32_t B = 1u << blog;
-  uint32_t* lpos = sh;                // [MX + 64]: entry MX + l is lane l's slot, entry x < MX slot i0 - MX + x
-  uint32_t* lkey = sh + MX + 64;
+  uint32_t* lent = sh;                  // [MX + 64]: entry MX + l is lane l's slot (tag << blog | position), entry
+                                        // x < MX slot i0 - MX + x; NOPOS for none
+  uint32_t* lneed = sh + MX + 64;       // [MX + 64]: the slot's window is some lane's candidate
   uint32_t* lwin = sh + 2 * (MX + 64);  // [MX + 64][NW]
   const bool xact = lane < MX && i0 + lane >= MX;
-  uint32_t p = i < count ? sb[i] & (B - 1u) : 0u;
-  const uint32_t xp = xact ? sb[i0 + lane - MX] & (B - 1u) : 0u;
+  const uint32_t e = i < count ? sb[i] : NOPOS;
+  uint32_t p = e & (B - 1u);
   const bool act = i < count && p < count;  // (p < count always with a correct sort; keeps every read in the block)
   if (!act) p = 0;
+  const uint32_t xe = xact ? sb[i0 + lane - MX] : NOPOS;
+  const uint32_t xp = xe & (B - 1u);
+  lent[MX + lane] = act ? e : NOPOS;
+  lneed[MX + lane] = 0;
+  if (lane < MX) {
+    lent[lane] = xact && xp < count ? xe : NOPOS;
+    lneed[lane] = 0;
+  }
+  wave_fence();
+  // which of the slots before this one are candidates (bit k - 1 for slot i - k), from the sorted entries alone: a
+  // slot's tag is its key's bits below the bucket's, so the low 12 bits of the tags are those of the keys
+  const uint32_t tag = e >> blog;
+  uint32_t okm = 0;
+#pragma unroll
+  for (uint32_t k = 1; k <= MX; ++k) {
+    if (k > ncand) break;  // (uniform)
+    const uint32_t ce = lent[MX + lane - k];
+    const bool ok = act && ce != NOPOS && (((ce >> blog) ^ tag) & 0xFFFu) == 0 && (ce & (B - 1u)) < p;
+    okm |= ok ? 1u << (k - 1) : 0u;
+    if (ok) lneed[MX + lane - k] = 1u;  // (lanes that share a candidate write the same value)
+  }
+  wave_fence();
+  // windows only where they are compared: a lane with a candidate, a slot that is one (about 7 in 10 of the
+  // positions of LZ-like data; the others' gathers are not made)
+  const bool want = act && (okm != 0u || lneed[MX + lane] != 0u);
+  const bool xwant = lane < MX && lneed[lane] != 0u;
   uint32_t w[NW], xw[NW];
-  load_window<NW>(inb, n, p, w);
+#pragma unroll
+  for (int q = 0; q < NW; ++q) w[q] = xw[q] = 0;
+  if (want) load_window<NW>(inb, n, p, w);
+  if (xwant) load_window<NW>(inb, n, xp < count ? xp : 0u, xw);
   // (the byte before p is read only where the window starts with 4 equal bytes: those lanes alone take the load)
   bool run = false;
-  if (act && p > 0 && w[0] == (w[0] & 0xFFu) * 0x01010101u) run = inb[p - 1] == (w[0] & 0xFFu);
-  if (lane < MX) load_window<NW>(inb, n, xp < count ? xp : 0u, xw);
-  const uint32_t key = window_key<NW>(w, keybits);
-  lpos[MX + lane] = act ? p : NOPOS;
-  lkey[MX + lane] = key;
+  if (okm && p > 0 && w[0] == (w[0] & 0xFFu) * 0x01010101u) run = inb[p - 1] == (w[0] & 0xFFu);
+  if (want) {
 #pragma unroll
-  for (int q = 0; q < NW; ++q) lwin[(MX + lane) * NW + q] = w[q];
-  if (lane < MX) {
-    lpos[lane] = xact ? xp : NOPOS;
-    lkey[lane] = window_key<NW>(xw, keybits);
+    for (int q = 0; q < NW; ++q) lwin[(MX + lane) * NW + q] = w[q];
+  }
+  if (xwant) {
 #pragma unroll
     for (int q = 0; q < NW; ++q) lwin[lane * NW + q] = xw[q];
   }
   wave_fence();
   const uint32_t maxlen = min(lencap, n - p);
-  // which of the slots before this one are candidates: bit k - 1 for slot i - k
-  uint32_t okm = 0;
-#pragma unroll
-  for (uint32_t k = 1; k <= MX; ++k) {
-    if (k > ncand) break;  // (uniform)
-    const uint32_t idx = MX + lane - k;
-    const uint32_t cp = lpos[idx], ck = lkey[idx];
-    const bool ok = act && cp != NOPOS && ((ck ^ key) & 0xFFFu) == 0 && cp < p;
-    okm |= ok ? 1u << (k - 1) : 0u;
-  }
   uint32_t best = 0, bestd = 0;
   if (run && okm) {  // a run of one byte value (kernels/zero.cl:219-222): 5 equal bytes from p - 1
     uint32_t r[NW];
@@ -555,7 +572,7 @@ __device__ inline void match_slots(const uint8_t* __restrict__ inb, uint32_t n, 
   for (uint32_t k = 1; k <= MX; ++k) {
     if (k > ncand) break;  // (uniform)
     const uint32_t idx = MX + lane - k;
-    const uint32_t cp = lpos[idx];
+    const uint32_t cp = lent[idx] & (B - 1u);
     uint32_t c[NW];
 #pragma unroll
     for (int q = 0; q < NW; ++q) c[q] = lwin[idx * NW + q];
@@ -564,8 +581,7 @@ __device__ inline void match_slots(const uint8_t* __restrict__ inb, uint32_t n, 
     best = better ? l : best;
     bestd = better ? p - cp : bestd;
   }
-  if (act) mb[p] = best >= 4 ? (best << 20) | (bestd - 1) : 0u;
-  else if (i >= count && i < n) mb[i] = 0;  // (the last 3 positions are never in srt: exactly these slots)
+  if (act && best >= 4) mb[p] = (best << 20) | (bestd - 1);  // (the others stay 0: the array was cleared)
 }
 
 // 256 sorted slots of block b, from slot s0 (a wavefront takes 64 of them)
@@ -584,9 +600,9 @@ __device__ __forceinline__ void enc_match_slots(const Block* __restrict__ blocks
   const uint32_t* sb = srt + ((size_t)b << blog);
   uint32_t* mb = match + ((size_t)b << blog);
   if (lencap <= 16)
-    match_slots<4>(inb, n, count, blog, keybits, min(ncand, MX), lencap, sb, i0, mb, sh[threadIdx.x >> 6]);
+    match_slots<4>(inb, n, count, blog, min(ncand, MX), lencap, sb, i0, mb, sh[threadIdx.x >> 6]);
   else
-    match_slots<8>(inb, n, count, blog, keybits, min(ncand, MX), min(lencap, 32u), sb, i0, mb, sh[threadIdx.x >> 6]);
+    match_slots<8>(inb, n, count, blog, min(ncand, MX), min(lencap, 32u), sb, i0, mb, sh[threadIdx.x >> 6]);
 }
 __global__ __launch_bounds__(256) void k_enc_match(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
                                                    uint32_t blog, uint32_t keybits, uint32_t ncand, uint32_t lencap,

@@ -143,6 +143,16 @@ __global__ __launch_bounds__(1024) void k_plan_sort_l(const PTab* __restrict__ t
     enc_sort_l_range(g.blog, g.hbits, g.srt, g.mt, g.bstart, g.q, blockIdx.x, gridDim.x);
   }
 }
+// every segment's match array to zero (the match stage writes only the positions that get a match)
+__global__ __launch_bounds__(256) void k_plan_clear(const PTab* __restrict__ t) {
+  for (uint32_t c = 0; c < t->nseg; ++c) {
+    const PSeg& g = t->s[c];
+    uint4* m = (uint4*)g.mt;  // (256-byte aligned: every arena piece is)
+    const size_t n16 = ((size_t)g.count << g.blog) / 4;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += (size_t)gridDim.x * 256)
+      m[i] = uint4{0u, 0u, 0u, 0u};
+  }
+}
 __global__ __launch_bounds__(256) void k_plan_match(const PTab* __restrict__ t, uint32_t* ctr, const Block* enc,
                                                     const uint8_t* in, uint32_t ncand, uint32_t lencap) {
   // one item a claim: the items in flight on an XCD then cover a fraction of one block, whose scattered match writes
@@ -420,6 +430,7 @@ static int plan_launch(zgpu_plan* p, int op, const uint8_t* d_src, uint8_t* d_ds
       mark();
       k_plan_sort_l<<<plan_grid(p, P_SORT_L), 1024, 0, st>>>(t);
       mark();
+      k_plan_clear<<<(unsigned)std::max(8, p->g->props.cu_count) * 8, 256, 0, st>>>(t);
       k_plan_match<<<plan_grid(p, P_MATCH), 256, 0, st>>>(t, q, p->d_enc, d_src, ncand, lencap);
       mark();
       k_plan_parse<<<plan_grid(p, P_PARSE), pass_threads_parse(p, ps), pass_lds(p, ps, parse_lds_bytes), st>>>(
