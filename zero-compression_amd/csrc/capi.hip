@@ -183,7 +183,7 @@ struct EncShape {
 // Fewer, longer segments cost more serial walking; more, shorter ones cost more joins (a long match crosses many).
 uint32_t walkers_of(uint32_t blog) {
   uint32_t w = std::min<uint32_t>(std::max<uint32_t>(PARSE_WALKERS_MIN, (1u << blog) / PARSE_SPAN), (1u << blog) / 32);
-  while (w > 1 && ((size_t)(1u << blog) / 32 + 19 * (size_t)w) * 4 > 160 * 1024) w >>= 1;
+  while (w > 1 && parse_lds_words(blog, w) * 4 > 160 * 1024) w >>= 1;
   return w;
 }
 EncShape default_shape(uint32_t blog) { return {blog, hbits_of(blog), hbits_of(blog) + KEY_TAG_BITS, walkers_of(blog)}; }
@@ -425,7 +425,6 @@ static int encode_run(zgpu_t* g, const uint8_t* d_in, const Block* blocks, uint3
                       uint8_t* d_out, uint32_t* d_out_len, uint8_t* base, hipStream_t st, bool prof) {
   const uint32_t blog = S.blog;
   const EncLayout L = enc_layout(nb, blog, S.hbits);
-  const Geo G = geo_of(blog);
   const uint32_t ncand = level <= 2 ? 2 : level <= 6 ? 4 : 8;
   const uint32_t lencap = level <= 6 ? 16 : 32;
   const uint32_t HB = 1u << S.hbits;
@@ -454,7 +453,7 @@ static int encode_run(zgpu_t* g, const uint8_t* d_in, const Block* blocks, uint3
                                                                                               ((size_t)nb << blog) / 4);
   k_enc_match<<<posgroups, 256, 0, st>>>(blocks, d_in, blog, S.keybits, ncand, lencap, srt, mt);
   if (prof) stage(g, st, "match");
-  const size_t parse_lds = ((size_t)G.B / 32 + 19 * S.walkers) * 4;
+  const size_t parse_lds = parse_lds_words(blog, S.walkers) * 4;
   k_enc_parse<<<nb, std::max(64u, S.walkers), parse_lds, st>>>(blocks, d_in, blog, S.walkers, lencap, mt, (uint32_t*)(base + L.mark));
   if (prof) stage(g, st, "parse");
   k_enc_tokens<<<nb, 64, 0, st>>>(blocks, d_in, blog, lencap, mt, (uint32_t*)(base + L.mark), T);

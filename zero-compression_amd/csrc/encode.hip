@@ -24,6 +24,11 @@ namespace zk {
 
 #define KEY_TAG_BITS 12u
 __host__ __device__ inline uint32_t hbits_of(uint32_t blog) { return blog - 3u < 15u ? blog - 3u : 15u; }
+// the parse's LDS words for a block of 2^blog bytes and `walkers` segments: the bitmap, then per walker its exit,
+// entry, meeting point and a 16-word window
+__host__ __device__ inline size_t parse_lds_words(uint32_t blog, uint32_t walkers) {
+  return ((size_t)1 << blog) / 32 + 19 * (size_t)walkers;
+}
 constexpr uint32_t PARSE_SPAN = 1024;       // default parse segment: positions per walker
 constexpr uint32_t PARSE_WALKERS_MIN = 128;  // default walkers per block, at least (each segment still >= 32 positions)
 

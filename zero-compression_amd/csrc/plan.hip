@@ -241,10 +241,7 @@ EncShape class_shape(uint32_t blog, uint32_t key_log) {
   S.walkers = walkers_of(blog);
   return S;
 }
-size_t parse_lds_bytes(const EncShape& S) {
-  const size_t B = (size_t)1 << S.blog;
-  return (B / 32 + 19 * (size_t)S.walkers) * 4;
-}
+size_t parse_lds_bytes(const EncShape& S) { return parse_lds_words(S.blog, S.walkers) * 4; }
 size_t hash_lds_bytes(const EncShape& S) {
   return (((size_t)1 << S.hbits) + 16 + 17 * ((size_t)1 << pbits_of(S.blog, S.hbits))) * 4;
 }
