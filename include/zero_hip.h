@@ -181,6 +181,9 @@ int zgpu_plan_set_stage_occupancy(zgpu_plan_t* p, const uint32_t* occ, int n);
 /* the slab bytes a pass holds at most (2 GiB unless set; at least 1 MiB): the encoder's scratch arena is about 19
  * times that */
 int zgpu_plan_pass_bytes(zgpu_plan_t* p, uint64_t slab_bytes);
+/* the match stage's threads per workgroup (64, 128, 256, 512 or 1024; 256 unless set or tuned), which are also the
+ * sorted slots one of its work items holds; threads = 0 only reads it. Returns the current value. */
+int zgpu_plan_match_workgroup(zgpu_plan_t* p, uint32_t threads);
 /* on: the next encode or decode records an event after every stage of every pass (graphs are not used meanwhile);
  * zgpu_plan_stage_ms then gives each stage's time summed over the passes (ms[stage], the stage order above) */
 int zgpu_plan_profile(zgpu_plan_t* p, int on);

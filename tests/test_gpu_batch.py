@@ -30,11 +30,13 @@ def files(seed):
     return out
 
 
-def run_plan(gpu, data, sizes, bl, level, streams=4, tune=False, graphs=0, pass_bytes=None):
+def run_plan(gpu, data, sizes, bl, level, streams=4, tune=False, graphs=0, pass_bytes=None, match_wg=0):
     src, rl, fi = pyzero.file_blocks(sizes, bl)
     with pyzero.Plan(gpu, src, rl, bl, streams) as plan:
         if pass_bytes:
             plan.set_pass_bytes(pass_bytes)
+        if match_wg:
+            assert plan.match_workgroup(match_wg) == match_wg
         d_in = gpu.alloc(max(1, data.size))
         d_in.upload(data)
         d_slabs = gpu.alloc(max(1, plan.slab_bytes))
@@ -56,6 +58,7 @@ def run_plan(gpu, data, sizes, bl, level, streams=4, tune=False, graphs=0, pass_
         run_plan.passes = plan.passes
         run_plan.slab_bytes = plan.slab_bytes
         run_plan.occupancy = plan.stage_occupancy()
+        run_plan.match_wg = plan.match_workgroup()
         # decode through the plan: payloads back to back
         blocks = results[-1]
         comp = [p if m else data[int(s): int(s) + int(r)].tobytes() for (m, p), s, r in zip(blocks, src, rl)]
@@ -97,7 +100,8 @@ def test_plan_equals_plain_path(gpu, bl, level):
 
 def test_plan_tuned_and_graph_replays(gpu):
     """the persistent stages' output does not depend on the tuned parameters (bucket bits, parse walkers, workgroups
-    per compute unit), on graph replays, or on how the blocks are cut into passes"""
+    per compute unit, the match stage's workgroup size), on graph replays, or on how the blocks are cut into
+    passes"""
     fs = files(7)
     sizes = [f.size for f in fs]
     data = np.concatenate(fs)
@@ -106,10 +110,15 @@ def test_plan_tuned_and_graph_replays(gpu):
     tuned, dec, classes, _ = run_plan(gpu, data, sizes, 18, 5, streams=3, tune=True, graphs=3)
     assert len(tuned) == 3 and len(dec) == 3
     assert set(run_plan.occupancy) >= {"hash", "match", "parse", "dlz"}
+    assert run_plan.match_wg in (64, 128, 256, 512, 1024)
     for r in tuned:
         assert r == base[0]
     for d in dec:
         assert np.array_equal(d, data)
+    for w in (64, 1024):
+        other, dec, _, _ = run_plan(gpu, data, sizes, 18, 5, match_wg=w)
+        assert run_plan.match_wg == w and other[0] == base[0], w
+        assert np.array_equal(dec[0], data)
     # many passes: a class split over passes, passes of several classes, graph replays of them
     many, dec, _, _ = run_plan(gpu, data, sizes, 18, 5, graphs=2, pass_bytes=1 << 20)
     assert run_plan.passes >= max(2, run_plan.slab_bytes >> 20), (run_plan.passes, run_plan.slab_bytes)

@@ -468,6 +468,7 @@ __global__ __launch_bounds__(1024) void k_enc_sort_l(uint32_t blog, uint32_t hbi
 // candidate is compared there, window against window.
 constexpr uint32_t MX = 8;             // most candidates of any level
 constexpr uint32_t NOPOS = 0xFFFFFFFFu;  // an LDS entry with no slot behind it
+constexpr uint32_t MATCH_LDS_WORDS = (MX + 64) * 10;  // a wavefront's entries, flags and windows (NW <= 8)
 
 template <int NW>
 __device__ inline void load_window(const uint8_t* inb, uint32_t n, uint32_t p, uint32_t (&w)[NW]) {
@@ -593,8 +594,7 @@ __device__ inline void match_slots(const uint8_t* __restrict__ inb, uint32_t n, 
 __device__ __forceinline__ void enc_match_slots(const Block* __restrict__ blocks, const uint8_t* __restrict__ in, uint32_t blog,
                                        uint32_t keybits, uint32_t ncand, uint32_t lencap,
                                        const uint32_t* __restrict__ srt, uint32_t* __restrict__ match, uint32_t b,
-                                       uint32_t s0) {
-  __shared__ uint32_t sh[4][(MX + 64) * 10];
+                                       uint32_t s0, uint32_t* sh) {  // sh: MATCH_LDS_WORDS per wavefront
   const uint32_t i0 = s0 + (threadIdx.x & ~63u);
   const Block blk = blocks[b];
   const uint32_t n = enc_len(blk, blog);
@@ -605,16 +605,18 @@ __device__ __forceinline__ void enc_match_slots(const Block* __restrict__ blocks
   const uint32_t* sb = srt + ((size_t)b << blog);
   uint32_t* mb = match + ((size_t)b << blog);
   if (lencap <= 16)
-    match_slots<4>(inb, n, count, blog, min(ncand, MX), lencap, sb, i0, mb, sh[threadIdx.x >> 6]);
+    match_slots<4>(inb, n, count, blog, min(ncand, MX), lencap, sb, i0, mb, sh + (threadIdx.x >> 6) * MATCH_LDS_WORDS);
   else
-    match_slots<8>(inb, n, count, blog, min(ncand, MX), min(lencap, 32u), sb, i0, mb, sh[threadIdx.x >> 6]);
+    match_slots<8>(inb, n, count, blog, min(ncand, MX), min(lencap, 32u), sb, i0, mb,
+                   sh + (threadIdx.x >> 6) * MATCH_LDS_WORDS);
 }
 __global__ __launch_bounds__(256) void k_enc_match(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
                                                    uint32_t blog, uint32_t keybits, uint32_t ncand, uint32_t lencap,
                                                    const uint32_t* __restrict__ srt, uint32_t* __restrict__ match) {
+  __shared__ uint32_t sh[4 * MATCH_LDS_WORDS];
   const uint32_t wg = xcd_swizzle(blockIdx.x, gridDim.x);
   enc_match_slots(blocks, in, blog, keybits, ncand, lencap, srt, match, wg >> (blog - 8),
-                  (wg << 8) & ((1u << blog) - 1u));
+                  (wg << 8) & ((1u << blog) - 1u), sh);
 }
 
 // ---------------------------------------------------------------- 4. lazy matching (kernels/zero.cl:239-255)

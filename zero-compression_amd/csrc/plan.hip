@@ -153,13 +153,17 @@ __global__ __launch_bounds__(256) void k_plan_clear(const PTab* __restrict__ t) 
       m[i] = uint4{0u, 0u, 0u, 0u};
   }
 }
-__global__ __launch_bounds__(256) void k_plan_match(const PTab* __restrict__ t, uint32_t* ctr, const Block* enc,
-                                                    const uint8_t* in, uint32_t ncand, uint32_t lencap) {
+// an item is blockDim.x sorted slots (64..1024, the tuner's choice; blocks have 1024 or more), one wavefront per 64;
+// dynamic LDS: MATCH_LDS_WORDS per wavefront
+__global__ __launch_bounds__(1024) void k_plan_match(const PTab* __restrict__ t, uint32_t* ctr, const Block* enc,
+                                                     const uint8_t* in, uint32_t ncand, uint32_t lencap) {
+  extern __shared__ uint32_t msh[];
+  const uint32_t lw = (uint32_t)__builtin_ctz(blockDim.x);
   // one item a claim: the items in flight on an XCD then cover a fraction of one block, whose scattered match writes
   // stay in that XCD's L2 until whole lines are written (16 a claim spread them over several blocks: twice the time)
-  PLAN_LOOP(P_MATCH, [](const PSeg& g) { return 1u << (g.blog - 8u); }, 1u) {
+  PLAN_LOOP(P_MATCH, [lw](const PSeg& g) { return 1u << (g.blog - lw); }, 1u) {
     const PSeg& g = t->s[c];
-    enc_match_slots(enc + g.first, in, g.blog, g.keybits, ncand, lencap, g.srt, g.mt, bl, sub << 8);
+    enc_match_slots(enc + g.first, in, g.blog, g.keybits, ncand, lencap, g.srt, g.mt, bl, sub << lw, msh);
   }
 }
 __global__ __launch_bounds__(1024) void k_plan_parse(const PTab* __restrict__ t, uint32_t* ctr, const Block* enc,
@@ -277,6 +281,7 @@ struct zgpu_plan {
   uint32_t* d_tmp = nullptr;
   uint32_t* d_ctr = nullptr;  // queue heads [P_STAGES][8] (QSTRIDE apart), then the sort queues' counts [MAX_SEGS][4]
   uint32_t occ[P_STAGES];
+  uint32_t match_wg = 256;  // threads per workgroup of the match stage (tuned)
   uint64_t pass_slab = PASS_SLAB;
   bool tables_ready = false, use_graphs = false;
   std::map<std::tuple<int, const void*, const void*, const void*, int>, hipGraphExec_t> graphs;
@@ -428,7 +433,8 @@ static int plan_launch(zgpu_plan* p, int op, const uint8_t* d_src, uint8_t* d_ds
       k_plan_sort_l<<<plan_grid(p, P_SORT_L), 1024, 0, st>>>(t);
       mark();
       k_plan_clear<<<(unsigned)std::max(8, p->g->props.cu_count) * 8, 256, 0, st>>>(t);
-      k_plan_match<<<plan_grid(p, P_MATCH), 256, 0, st>>>(t, q, p->d_enc, d_src, ncand, lencap);
+      k_plan_match<<<plan_grid(p, P_MATCH), p->match_wg, (p->match_wg / 64) * MATCH_LDS_WORDS * 4, st>>>(
+          t, q, p->d_enc, d_src, ncand, lencap);
       mark();
       k_plan_parse<<<plan_grid(p, P_PARSE), pass_threads_parse(p, ps), pass_lds(p, ps, parse_lds_bytes), st>>>(
           t, q, p->d_enc, d_src, lencap);
@@ -594,6 +600,16 @@ int zgpu_plan_set_stage_occupancy(zgpu_plan_t* p, const uint32_t* occ, int n) {
   return ZGPU_OK;
 }
 
+int zgpu_plan_match_workgroup(zgpu_plan_t* p, uint32_t threads) {
+  if (threads) {
+    if (threads < 64 || threads > 1024 || (threads & (threads - 1)))
+      return fail(p->g, ZGPU_ERR_ARG, "match workgroup: 64, 128, 256, 512 or 1024 threads");
+    p->match_wg = threads;
+    plan_drop_graphs(p);
+  }
+  return (int)p->match_wg;
+}
+
 int zgpu_plan_pass_bytes(zgpu_plan_t* p, uint64_t slab_bytes) {
   if (slab_bytes < (1u << 20)) return fail(p->g, ZGPU_ERR_ARG, "a pass holds at least 1 MiB of slabs");
   p->pass_slab = slab_bytes;
@@ -735,6 +751,33 @@ int zgpu_plan_tune(zgpu_plan_t* p, const uint8_t* d_in, int level, uint8_t* d_sl
   }
   for (uint32_t s = 0; s < 10; ++s) p->occ[s] = best_occ[s];
   for (uint32_t s = 10; s < P_STAGES; ++s) p->occ[s] = DEFAULT_OCC[s];
+  // the match stage's workgroup size, at the number of wavefronts per CU found best for 256 threads
+  {
+    const uint32_t occ256 = p->match_wg == 256 ? p->occ[P_MATCH] : p->occ[P_MATCH] * p->match_wg / 256;
+    uint32_t best_w = p->match_wg, best_o = p->occ[P_MATCH];
+    float best = 1e30f;
+    for (uint32_t w : {64u, 128u, 256u, 512u, 1024u}) {
+      if (rc != ZGPU_OK) break;
+      p->match_wg = w;
+      p->occ[P_MATCH] = std::max<uint32_t>(1, std::min<uint32_t>(32, occ256 * 256 / w));
+      for (int rep = 0; rep < 2 && rc == ZGPU_OK; ++rep) {
+        (void)hipEventRecord(p->stage_ev[0], g->stream);
+        rc = plan_launch(p, 0, d_in, d_slabs, d_out_len, level, p->stage_ev.data() + 1);
+        if (rc == ZGPU_OK && hipEventSynchronize(p->stage_ev[nev - 1]) != hipSuccess)
+          rc = fail(g, ZGPU_ERR_DEVICE, "tune: the match stage failed");
+        if (rc != ZGPU_OK || rep == 0) continue;
+        float ms = 0;
+        for (size_t ps = 0; ps < p->passes.size(); ++ps) {
+          float m = 0;
+          (void)hipEventElapsedTime(&m, p->stage_ev[ps * 10 + P_MATCH], p->stage_ev[ps * 10 + P_MATCH + 1]);
+          ms += m;
+        }
+        if (ms < best) { best = ms; best_w = w; best_o = p->occ[P_MATCH]; }
+      }
+    }
+    p->match_wg = best_w;
+    p->occ[P_MATCH] = best_o;
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   plan_drop_graphs(p);

@@ -40,6 +40,7 @@ EXPORTS = (
     "zgpu_plan_create", "zgpu_plan_destroy", "zgpu_plan_slab_bytes", "zgpu_plan_slab_offsets", "zgpu_plan_classes",
     "zgpu_plan_encode", "zgpu_plan_set_stored", "zgpu_plan_decode", "zgpu_plan_tune", "zgpu_plan_use_graphs",
     "zgpu_plan_stage_occupancy", "zgpu_plan_set_stage_occupancy", "zgpu_plan_passes", "zgpu_plan_pass_bytes",
+    "zgpu_plan_match_workgroup",
     "zgpu_plan_profile", "zgpu_plan_stage_ms",
     "zgpu_serve_bench",
 )
@@ -103,6 +104,7 @@ def load_library(path=LIB_PATH):
         "zgpu_plan_use_graphs": (i32, [vp, i32]),
         "zgpu_plan_stage_occupancy": (i32, [vp, vp, i32]), "zgpu_plan_passes": (i32, [vp]),
         "zgpu_plan_pass_bytes": (i32, [vp, u64]), "zgpu_plan_set_stage_occupancy": (i32, [vp, vp, i32]),
+        "zgpu_plan_match_workgroup": (i32, [vp, u32]),
         "zgpu_plan_profile": (i32, [vp, i32]), "zgpu_plan_stage_ms": (i32, [vp, vp, i32]),
         "zgpu_serve_bench": (i32, [vp, ctypes.POINTER(ServeCfg), vp, ctypes.c_uint64, vp, ctypes.c_uint64,
                                    ctypes.POINTER(ServeReport)]),
@@ -481,6 +483,13 @@ class Plan:
     def set_pass_bytes(self, n):
         """at most n slab bytes per pass (the encoder's scratch is about 19 n)"""
         self.gpu._check(self.lib.zgpu_plan_pass_bytes(self.h, int(n)))
+
+    def match_workgroup(self, threads=0):
+        """the match stage's threads per workgroup; sets it first when threads is 64..1024"""
+        r = int(self.lib.zgpu_plan_match_workgroup(self.h, int(threads)))
+        if r < 0:
+            self.gpu._check(r)
+        return r
 
 
 def file_blocks(sizes, block_log):
