@@ -408,13 +408,11 @@ def main():
     d_tab.upload(tab)
     slabs = gpu.alloc(n)
     lens = gpu.alloc(nb * 4)
-    batch = 4096  # blocks per encoder launch (1 GiB at 2^18)
+    batch = (1 << 30) >> bl  # blocks of the profiled encoder launch (1 GiB): its stages and its counters
 
     def encode_pass(din, dtab, k_blocks, out_slabs, out_lens):
-        for b0 in range(0, k_blocks, batch):
-            k = min(batch, k_blocks - b0)
-            gpu.encode_blocks(din.ptr, dtab.addr(b0 * 32), k, bl, level, out_slabs.addr(b0 << bl),
-                              out_lens.addr(b0 * 4))
+        # one call over every block (the library launches at most 2 GiB of blocks at a time)
+        gpu.encode_blocks(din.ptr, dtab.ptr, k_blocks, bl, level, out_slabs.ptr, out_lens.ptr)
 
     ev0, ev1 = gpu.event(), gpu.event()
     # ---- encode

@@ -98,7 +98,8 @@ int zgpu_event_elapsed_ms(zgpu_t* g, void* start, void* stop, float* ms);
 /* Encodes nb blocks (each raw_len <= 2^block_log). Block b's payload goes to out + (b << block_log) and its size to
  * out_len[b], or ZGPU_STORED when it would not be smaller than the block. The kernels check the table themselves: an
  * entry with raw_len > 2^block_log is read as an empty block and gets out_len ZGPU_BAD_BLOCK, and nothing is read
- * or written outside its slab. */
+ * or written outside its slab. More than 2^31 bytes of blocks (8192 of 2^18) run as several launches in turn, over
+ * scratch for one of them. */
 int zgpu_encode_blocks(zgpu_t* g, const uint8_t* d_in, const zgpu_block_t* d_blocks, uint32_t nb, uint32_t block_log,
                        int level, uint8_t* d_out, uint32_t* d_out_len, zgpu_stream_t s);
 /* Decodes nb blocks into out; err[b] receives ZGPU_E_* bits (0 = good). Asynchronous: read err after syncing. The

@@ -33,3 +33,7 @@ def test_scratch_sizes():
         d = lib.zgpu_decode_scratch_bytes(nb, bl)
         assert 10 * (nb << bl) < e < 24 * (nb << bl)
         assert (nb << bl) < d < 3 * (nb << bl)
+        # (the encoder runs at most 2^31 bytes of blocks per launch, over scratch for one launch)
+        per = (1 << 31) >> bl
+        assert lib.zgpu_encode_scratch_bytes(4 * per, bl) == lib.zgpu_encode_scratch_bytes(per, bl)
+        assert lib.zgpu_encode_scratch_bytes(per, bl) > lib.zgpu_encode_scratch_bytes(per // 2, bl)

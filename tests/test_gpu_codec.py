@@ -236,6 +236,38 @@ def test_stored_and_method_errors(gpu):
         gpu.decompress([(2, 10, b"0123456789")], 16)
 
 
+def test_encode_past_one_launch(gpu):
+    """more than 2^31 bytes in one zgpu_encode_blocks call (8193 blocks of 2^18) runs as several launches and gives
+    every block the payload it gets alone: the blocks are 128 copies of 64 blocks, so each must match its copy's
+    payload from a small call"""
+    bl, level = 18, 5
+    B = 1 << bl
+    base = pyzero.gen_lz(5, 64 * B)
+    nb = 128 * 64 + 1
+    din = gpu.alloc(nb * B)
+    for k in range(128):
+        din.upload(base, offset=k * 64 * B)
+    din.upload(base[:B], offset=128 * 64 * B)
+    src = np.arange(nb, dtype=np.uint64) * B
+    tab = pyzero.block_table([B] * nb, src=src, dst=src)
+    dtab = gpu.alloc(tab.nbytes)
+    dtab.upload(tab)
+    dout = gpu.alloc(nb * B)
+    dlen = gpu.alloc(nb * 4)
+    gpu.encode_blocks(din.ptr, dtab.ptr, nb, bl, level, dout.ptr, dlen.ptr)
+    gpu.sync()
+    lens = dlen.download(dtype=np.uint32)
+    small = gpu.compress(base.tobytes(), bl, level)
+    want = [len(p) if m else pyzero.STORED for m, p in small]
+    assert [int(x) for x in lens[:64]] == want
+    assert all((lens[k * 64: (k + 1) * 64] == lens[:64]).all() for k in range(128))
+    assert int(lens[-1]) == want[0]
+    for b in (0, 63, 8191, 8192):  # (either side of the launch boundary, and the last block)
+        assert dout.download(int(lens[b]), offset=b * B).tobytes() == small[b % 64][1]
+    for buf in (din, dtab, dout, dlen):
+        buf.free()
+
+
 def test_large_roundtrip_device_resident(gpu):
     """64 MiB of the LZ workload through device-resident encode and decode; a sample decoded by the oracle"""
     bl, level = 18, 5

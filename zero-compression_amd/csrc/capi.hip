@@ -254,8 +254,11 @@ uint32_t host_mulmod(uint32_t a, uint32_t b) {
 
 extern "C" {
 
+// blocks per encoder launch at most: 2^31 positions (2 GiB of input), so that no grid exceeds 2^32 threads (the
+// match kernel runs one thread per position); larger calls run as several launches over the same scratch
+static uint32_t enc_launch_blocks(uint32_t blog) { return 1u << (31 - blog); }
 size_t zgpu_encode_scratch_bytes(uint32_t nb, uint32_t block_log) {
-  return enc_layout(nb, block_log, hbits_of(block_log)).total;
+  return enc_layout(std::min(nb, enc_launch_blocks(block_log)), block_log, hbits_of(block_log)).total;
 }
 size_t zgpu_decode_scratch_bytes(uint32_t nb, uint32_t block_log) { return dec_layout(nb, block_log).total; }
 
@@ -451,6 +454,7 @@ static int encode_run(zgpu_t* g, const uint8_t* d_in, const Block* blocks, uint3
   // the match array cleared: the match kernel writes only the positions that get a match
   k_zero16<<<(unsigned)std::min<size_t>(((size_t)nb << blog) / 1024 + 1, 16384), 256, 0, st>>>((uint4*)mt,
                                                                                               ((size_t)nb << blog) / 4);
+  if (prof) stage(g, st, "clear");
   k_enc_match<<<posgroups, 256, 0, st>>>(blocks, d_in, blog, S.keybits, ncand, lencap, srt, mt);
   if (prof) stage(g, st, "match");
   const size_t parse_lds = parse_lds_words(blog, S.walkers) * 4;
@@ -472,8 +476,13 @@ int zgpu_encode_blocks(zgpu_t* g, const uint8_t* d_in, const zgpu_block_t* d_blo
   if (nb == 0) return ZGPU_OK;
   hipStream_t st = s ? (hipStream_t)s : g->stream;
   const EncShape S = default_shape(blog);
-  if (int rc = ensure(g, g->enc, enc_layout(nb, blog, S.hbits).total)) return rc;
-  return encode_run(g, d_in, (const Block*)d_blocks, nb, S, level, d_out, d_out_len, (uint8_t*)g->enc.p, st, true);
+  const uint32_t per = std::min(nb, enc_launch_blocks(blog));
+  if (int rc = ensure(g, g->enc, enc_layout(per, blog, S.hbits).total)) return rc;
+  for (uint32_t b0 = 0; b0 < nb; b0 += per)  // (in order on one stream: each launch's scratch is free for the next)
+    if (int rc = encode_run(g, d_in, (const Block*)d_blocks + b0, std::min(per, nb - b0), S, level,
+                            d_out + ((size_t)b0 << blog), d_out_len + b0, (uint8_t*)g->enc.p, st, b0 == 0))
+      return rc;
+  return ZGPU_OK;
 }
 
 // one stream section (u8 kind, u32 len, body) of n symbols by the block encoder's own kernels: the symbols are the
