@@ -1287,9 +1287,19 @@ __device__ __forceinline__ void enc_pack_block(const Block* __restrict__ blocks,
     const uint32_t data0 = 5 + 32 + 2 * np + 256;
     const uint32_t LC = lane_cap(G, s);
     const uint8_t* lb = R.lanes + (size_t)b * lanes_per_block(G) + lane_off(G, s);
-    for (uint32_t j = 0; j < 64; ++j) {
+    // the 64 lanes' bytes side by side, four threads to a lane, 16 bytes each per step (a lane's bytes end at the
+    // end of its LC-byte area, being written backwards)
+    for (uint32_t j = tid >> 2; j < 64; j += blockDim.x >> 2) {
       const uint32_t l = ll[j];
-      wg_copy(d + data0 + loff[j], lb + (size_t)(j + 1) * LC - l, l);
+      const uint8_t* src = lb + (size_t)(j + 1) * LC - l;
+      uint8_t* dst = d + data0 + loff[j];
+      const uint32_t full = l & ~15u;
+      for (uint32_t i = (tid & 3u) * 16u; i < full; i += 64u) {
+        uint32_t v[4];
+        __builtin_memcpy(v, src + i, 16);
+        __builtin_memcpy(dst + i, v, 16);
+      }
+      for (uint32_t i = full + (tid & 3u); i < l; i += 4u) dst[i] = src[i];
     }
     __syncthreads();
   }
