@@ -199,7 +199,8 @@ __device__ __forceinline__ void dec_stream(const Block* __restrict__ blocks, con
   if (kind != 1 || len < 32 + 4 * 64) { if (lane == 0) atomicOr(&err[b], E_STREAM); return; }
 
   __shared__ uint32_t fc[256];       // freq | cum << 16
-  __shared__ uint8_t slot[4096];     // slot -> symbol
+  __shared__ uint32_t slot32[1024];  // slot -> symbol (bytes, filled four at a time)
+  uint8_t* const slot = (uint8_t*)slot32;
   __shared__ uint8_t stg[512];       // decoded symbols on their way out
 
   // frequencies of the present symbols, in symbol order; lane k owns symbols 4k..4k+3
@@ -245,11 +246,18 @@ __device__ __forceinline__ void dec_stream(const Block* __restrict__ blocks, con
       const uint32_t mid = (lo + hi + 1) >> 1;
       if ((fc[mid] >> 16) <= i0) lo = mid; else hi = mid - 1;
     }
-    uint32_t sy = lo;
-    for (uint32_t k = 0; k < 64; ++k) {
-      const uint32_t i = i0 + k;
-      while ((fc[sy] >> 16) + (fc[sy] & 0xFFFFu) <= i) ++sy;
-      slot[i] = (uint8_t)sy;
+    uint32_t sy = lo, e = fc[sy], end = (e >> 16) + (e & 0xFFFFu);  // (end: where sy's slots stop)
+    for (uint32_t k = 0; k < 64; k += 4) {  // four slots to a 32-bit store
+      uint32_t w = 0;
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u) {
+        while (end <= i0 + k + u) {
+          e = fc[++sy];
+          end = (e >> 16) + (e & 0xFFFFu);
+        }
+        w |= sy << (8u * u);
+      }
+      slot32[(i0 + k) >> 2] = w;
     }
   }
   __syncthreads();
