@@ -455,7 +455,7 @@ static int encode_run(zgpu_t* g, const uint8_t* d_in, const Block* blocks, uint3
   k_zero16<<<(unsigned)std::min<size_t>(((size_t)nb << blog) / 1024 + 1, 16384), 256, 0, st>>>((uint4*)mt,
                                                                                               ((size_t)nb << blog) / 4);
   if (prof) stage(g, st, "clear");
-  k_enc_match<<<posgroups, 256, 0, st>>>(blocks, d_in, blog, S.keybits, ncand, lencap, srt, mt);
+  k_enc_match<<<posgroups, 256, 0, st>>>(blocks, d_in, blog, ncand, lencap, srt, mt);
   if (prof) stage(g, st, "match");
   const size_t parse_lds = parse_lds_words(blog, S.walkers) * 4;
   k_enc_parse<<<nb, std::max(64u, S.walkers), parse_lds, st>>>(blocks, d_in, blog, S.walkers, lencap, mt, (uint32_t*)(base + L.mark));

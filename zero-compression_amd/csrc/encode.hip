@@ -488,11 +488,6 @@ __device__ inline void load_window(const uint8_t* inb, uint32_t n, uint32_t p, u
   }
 }
 
-// the key of the position whose window is w (hash5 of its first 5 bytes, the fifth read as 0 past the block)
-template <int NW>
-__device__ inline uint32_t window_key(const uint32_t (&w)[NW], uint32_t keybits) {
-  return hash5v(w[0], w[1] & 0xFFu) >> (32 - keybits);
-}
 
 // the number of leading bytes where a and b agree (4 * NW when all do)
 template <int NW>
@@ -592,7 +587,7 @@ __device__ inline void match_slots(const uint8_t* __restrict__ inb, uint32_t n, 
 
 // 256 sorted slots of block b, from slot s0 (a wavefront takes 64 of them)
 __device__ __forceinline__ void enc_match_slots(const Block* __restrict__ blocks, const uint8_t* __restrict__ in, uint32_t blog,
-                                       uint32_t keybits, uint32_t ncand, uint32_t lencap,
+                                       uint32_t ncand, uint32_t lencap,
                                        const uint32_t* __restrict__ srt, uint32_t* __restrict__ match, uint32_t b,
                                        uint32_t s0, uint32_t* sh) {  // sh: MATCH_LDS_WORDS per wavefront
   const uint32_t i0 = s0 + (threadIdx.x & ~63u);
@@ -611,11 +606,11 @@ __device__ __forceinline__ void enc_match_slots(const Block* __restrict__ blocks
                    sh + (threadIdx.x >> 6) * MATCH_LDS_WORDS);
 }
 __global__ __launch_bounds__(256) void k_enc_match(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
-                                                   uint32_t blog, uint32_t keybits, uint32_t ncand, uint32_t lencap,
+                                                   uint32_t blog, uint32_t ncand, uint32_t lencap,
                                                    const uint32_t* __restrict__ srt, uint32_t* __restrict__ match) {
   __shared__ uint32_t sh[4 * MATCH_LDS_WORDS];
   const uint32_t wg = xcd_swizzle(blockIdx.x, gridDim.x);
-  enc_match_slots(blocks, in, blog, keybits, ncand, lencap, srt, match, wg >> (blog - 8),
+  enc_match_slots(blocks, in, blog, ncand, lencap, srt, match, wg >> (blog - 8),
                   (wg << 8) & ((1u << blog) - 1u), sh);
 }
 

@@ -163,7 +163,7 @@ __global__ __launch_bounds__(1024) void k_plan_match(const PTab* __restrict__ t,
   // stay in that XCD's L2 until whole lines are written (16 a claim spread them over several blocks: twice the time)
   PLAN_LOOP(P_MATCH, [lw](const PSeg& g) { return 1u << (g.blog - lw); }, 1u) {
     const PSeg& g = t->s[c];
-    enc_match_slots(enc + g.first, in, g.blog, g.keybits, ncand, lencap, g.srt, g.mt, bl, sub << lw, msh);
+    enc_match_slots(enc + g.first, in, g.blog, ncand, lencap, g.srt, g.mt, bl, sub << lw, msh);
   }
 }
 __global__ __launch_bounds__(1024) void k_plan_parse(const PTab* __restrict__ t, uint32_t* ctr, const Block* enc,
