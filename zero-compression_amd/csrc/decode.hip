@@ -448,9 +448,7 @@ __device__ inline void group_copy(uint8_t* o, const uint8_t* litb, uint32_t raw,
     const bool litc = w < nlit;
     const bool pat = !litc && D < 16u;
     uint32_t rep = 0, within = litc ? w : w - nlit;
-    if (ballot(live && !litc && D >= 16u && D < M)) {
-      if (!litc && D >= 16u && D < M) divmod_small(within, (D + 15u) >> 4, rep, within);
-    }
+    if (live && !litc && D >= 16u && D < M) divmod_small(within, (D + 15u) >> 4, rep, within);
     const uint32_t base = litc ? opk : ms + rep * D;
     const uint32_t y0 = base + 16u * within;
     const uint32_t cend = litc ? ms : pat ? me : min(me, base + D);
@@ -462,23 +460,17 @@ __device__ inline void group_copy(uint8_t* o, const uint8_t* litb, uint32_t raw,
     const bool straddle = live && !slow && from_out && s0 + need > opk;
     const uint8_t* a1 = from_out ? o + s0 : litb + lik + (s0 - opk);
     U128 v = ld128(live && !slow ? a1 : litb);
-    if (ballot(straddle)) {
-      if (straddle) v = splice(v, ld128(litb + lik), opk - s0);
-    }
-    if (ballot(live && !slow && pat)) {
-      if (live && !slow && pat) v = pattern16(v, mod_small(16u * within, D), D);
-    }
+    if (straddle) v = splice(v, ld128(litb + lik), opk - s0);
+    if (live && !slow && pat) v = pattern16(v, mod_small(16u * within, D), D);
     if (live && !slow) {
       if (len == 16u) st128(o + y0, v);
       else st_partial(o + y0, v, len);
     }
-    if (ballot(slow)) {
-      if (slow) {  // exact, byte by byte
-        const uint32_t ph = pat ? mod_small(16u * within, D) : 0u;
-        for (uint32_t i = 0; i < len; ++i) {
-          const uint32_t si = pat ? s0 + mod_small(ph + i, D) : s0 + i;
-          o[y0 + i] = si < opk ? o[si] : litb[lik + (si - opk)];
-        }
+    if (slow) {  // exact, byte by byte
+      const uint32_t ph = pat ? mod_small(16u * within, D) : 0u;
+      for (uint32_t i = 0; i < len; ++i) {
+        const uint32_t si = pat ? s0 + mod_small(ph + i, D) : s0 + i;
+        o[y0 + i] = si < opk ? o[si] : litb[lik + (si - opk)];
       }
     }
   }
@@ -507,13 +499,11 @@ __device__ inline void ring_write(uint8_t* ring, uint32_t y, const U128& v, uint
     else st_partial(ring + i, v, len);
   }
   const bool wrap = on && i + len > RING, mirror = on && i < 16u;
-  if (ballot(wrap || mirror)) {
-    if (wrap) st_partial(ring, shr128(v, RING - i), i + len - RING);  // (at most 15 bytes)
-    if (mirror) {
-      const uint32_t m = min(len, 16u - i);
-      if (m == 16u) __builtin_memcpy(ring + RING + i, &v, 16);
-      else st_partial(ring + RING + i, v, m);
-    }
+  if (wrap) st_partial(ring, shr128(v, RING - i), i + len - RING);  // (at most 15 bytes)
+  if (mirror) {
+    const uint32_t m = min(len, 16u - i);
+    if (m == 16u) __builtin_memcpy(ring + RING + i, &v, 16);
+    else st_partial(ring + RING + i, v, m);
   }
 }
 
@@ -521,6 +511,7 @@ __device__ inline void ring_write(uint8_t* ring, uint32_t y, const U128& v, uint
 struct CDesc {
   uint32_t y0, len, s0, flags;  // flags: group | kind << 8 | pattern << 10 | phase << 11 | D << 16 (D < 16 only)
   uint32_t k;                   // the token (index in the chunk of tokens)
+  uint32_t rend;                // the end of what it reads of this chunk of tokens' output (0: nothing)
 };
 enum : uint32_t { K_PRE = 0, K_RING = 1, K_SLOW = 2, K_NONE = 3 };
 
@@ -541,20 +532,18 @@ __device__ inline CDesc describe(uint32_t cj, bool live, uint32_t k, const TokF*
   uint32_t s0 = litc ? y0 : y0 - D;
   bool pat = false;
   uint32_t ph = 0;
-  if (ballot(live && !litc && D < M)) {  // a match that overlaps its output: the period pattern, or repetitions
-    if (live && !litc && D < M) {
-      if (D < 16u) {
-        pat = true;
-        s0 = ms - D;
-        ph = mod_small(16u * wm, D);
-      } else {
-        uint32_t rep, within;
-        divmod_small(wm, (D + 15u) >> 4, rep, within);
-        const uint32_t base = ms + rep * D;
-        y0 = base + 16u * within;
-        cend = min(cend, base + D);
-        s0 = y0 - (rep + 1u) * D;
-      }
+  if (live && !litc && D < M) {  // a match that overlaps its output: the period pattern, or repetitions
+    if (D < 16u) {
+      pat = true;
+      s0 = ms - D;
+      ph = mod_small(16u * wm, D);
+    } else {
+      uint32_t rep, within;
+      divmod_small(wm, (D + 15u) >> 4, rep, within);
+      const uint32_t base = ms + rep * D;
+      y0 = base + 16u * within;
+      cend = min(cend, base + D);
+      s0 = y0 - (rep + 1u) * D;
     }
   }
   const uint32_t len = live ? min(16u, cend - y0) : 0u;
@@ -572,6 +561,7 @@ __device__ inline CDesc describe(uint32_t cj, bool live, uint32_t k, const TokF*
   d.s0 = s0;
   d.flags = grp | kind << 8 | (pat ? 1u : 0u) << 10 | ph << 11 | (pat ? D : 0u) << 16;
   d.k = kk;
+  d.rend = kind == K_PRE || kind == K_NONE ? 0u : min(s0 + need, opk);
   return d;
 }
 
@@ -601,47 +591,48 @@ __device__ inline void ring_copy(uint8_t* ring, const uint8_t* o, const uint8_t*
     for (uint32_t r = 0; r < RROUNDS; ++r) {
       const uint32_t c0 = base + 64 * r;
       if (c0 >= C) {  // (uniform: no chunks left for this round)
-        d[r] = CDesc{0u, 0u, 0u, K_NONE << 8, 0u};
+        d[r] = CDesc{0u, 0u, 0u, K_NONE << 8, 0u, 0u};
         pre[r] = U128{0, 0};
         continue;
       }
+      // the token of chunk c0 + lane: the tokens that start at or before c0, plus those that start in slots 1..lane
       sown[lane] = 0;
       wave_fence();
-      const uint32_t jk = ccs_lane <= c0 ? 0u : ccs_lane - c0;
-      if (on && jk < 64) atomicMax(&sown[jk], lane + 1);
+      const uint32_t jk = ccs_lane - c0;  // (wraps for a token that starts before c0)
+      if (on && jk < 64u) sown[jk] = 1u;
       wave_fence();
-      const uint32_t k = min(scan_max(sown[lane]), 64u) - 1u;
+      const uint64_t starts = ballot(sown[lane] != 0u);
       wave_fence();
+      const uint32_t k = popc64(ballot(on && ccs_lane <= c0)) - 1u + mbcnt64(starts >> 1);
       const uint32_t cj = c0 + lane;
       const uint8_t* src;
       d[r] = describe(cj, cj < C, k, stk, o, litb, raw, lo, src);
       pre[r] = ld128(((d[r].flags >> 8) & 3u) == K_PRE ? src : litb);
     }
-    // in chunk order, which is group order: round by round, and inside a round group by group (a chunk reads only
-    // what earlier groups wrote, and a group's chunks in later rounds come after its chunks in this one)
+    // Round by round, and inside a round pass by pass: every chunk whose source bytes are all written goes, that is
+    // one that reads output before the first chunk not yet written (chunks are in output order, so every byte
+    // before that chunk is in the window), or its own literals, or older output in global memory. The group rule
+    // makes the first chunk not yet written always ready (it reads earlier groups' output or its own literals), so
+    // a round takes at most 64 passes, and usually about two: a pass is bounded by the dependences of the data,
+    // not by its groups. (The branches below are plain divergent ifs: the compiler skips a region that no lane
+    // enters, where an explicit ballot would cost two vector instructions.)
 #pragma unroll
     for (uint32_t r = 0; r < RROUNDS; ++r) {
       const uint32_t kind = (d[r].flags >> 8) & 3u;
-      const uint32_t grp = d[r].flags & 255u;
-      const bool live = kind != K_NONE;
-      if (!ballot(live)) break;
-      const uint32_t qa = wave_min(live ? grp : 255u), qb = wave_max(live ? grp : 0u);
-      for (uint32_t q = qa; q <= qb; ++q) {
-        const bool act = live && grp == q;
-        if (!ballot(act)) continue;
+      bool done = kind == K_NONE;
+      for (uint32_t it = 0; it < 64u; ++it) {
+        const uint64_t left = ballot(!done);
+        if (!left) break;
+        const uint32_t F = readlane(d[r].y0, (uint32_t)__builtin_ctzll(left));
+        const bool act = !done && d[r].rend <= F;
+        done = done || act;
         U128 v = pre[r];
-        if (ballot(act && kind == K_RING)) {
-          if (act && kind == K_RING) v = ring_read16(ring, d[r].s0);
-        }
+        if (act && kind == K_RING) v = ring_read16(ring, d[r].s0);
         const bool pat = (d[r].flags >> 10) & 1u;
-        if (ballot(act && pat && kind != K_SLOW)) {
-          if (act && pat && kind != K_SLOW) v = pattern16(v, (d[r].flags >> 11) & 15u, d[r].flags >> 16);
-        }
+        if (act && pat && kind != K_SLOW) v = pattern16(v, (d[r].flags >> 11) & 15u, d[r].flags >> 16);
         ring_write(ring, d[r].y0, v, d[r].len, act && kind != K_SLOW);
-        if (ballot(act && kind == K_SLOW)) {
-          if (act && kind == K_SLOW) copy_slow(d[r], ring, o, litb, lo, stk);
-        }
-        wave_fence();  // (this group's bytes are in the window before the next group reads them)
+        if (act && kind == K_SLOW) copy_slow(d[r], ring, o, litb, lo, stk);
+        wave_fence();  // (this pass's bytes are in the window before the next pass reads them)
       }
     }
     // (every load of this batch has been used; saying so keeps the compiler from waiting at the loop's head, which
@@ -871,7 +862,9 @@ __device__ __forceinline__ void dec_lz_block(const Block* __restrict__ blocks, c
   ring_flush(ring, o, fl_a, fl_e);
   wave_copy(o + op, litb + li, nl - li);
 }
-__global__ __launch_bounds__(64) void k_dec_lz(const Block* __restrict__ blocks, const uint8_t* __restrict__ comp,
+// at most 96 VGPRs, so that 5 wavefronts fit a SIMD (LDS allows 22 per CU): left to itself the compiler takes 97,
+// which leaves 4, and 16384 blocks of 2^18 then replay 7 % slower (5.26 against 4.90 ms)
+__global__ __launch_bounds__(64, 5) void k_dec_lz(const Block* __restrict__ blocks, const uint8_t* __restrict__ comp,
                                                uint32_t blog, const uint8_t* __restrict__ lit,
                                                const uint8_t* __restrict__ tok, const uint8_t* __restrict__ ext,
                                                const uint8_t* __restrict__ sym, const uint8_t* __restrict__ grp,

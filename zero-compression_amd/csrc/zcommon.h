@@ -62,6 +62,10 @@ __device__ inline uint32_t lane_id() { return __lane_id(); }
 __device__ inline uint64_t ballot(bool p) { return __ballot(p); }
 __device__ inline uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
 __device__ inline uint32_t popc64(uint64_t m) { return (uint32_t)__popcll(m); }
+// the set bits of m below this lane
+__device__ inline uint32_t mbcnt64(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
 
 // v in `lane` (the lane index must be the same in every lane: v_readlane)
 __device__ inline uint32_t readlane(uint32_t v, uint32_t lane) { return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane); }
