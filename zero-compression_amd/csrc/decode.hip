@@ -420,71 +420,16 @@ __device__ inline uint32_t chunk_count(uint32_t L, uint32_t M, uint32_t D) {
   return nlit + reps * ((D + 15u) >> 4) + ((rem + 15u) >> 4);
 }
 
-// One group's chunks [cb, ce) copied by the whole wavefront, chunk cb + 64r + j on lane j in round r. Lane k < 64
-// holds token k of the chunk of tokens (its first chunk ccs_lane, its group membership in_group); the token that
-// holds a lane's chunk is the prefix maximum over the chunk slots where tokens start. A chunk loads its 16 source
-// bytes at once (from the literal stream when they are the token's own literals, else from output written by an
-// earlier group), and stores exactly its length. Sources that straddle the token's first output byte take a second
-// load; a 16-byte read of output past the block's end goes byte by byte.
-__device__ inline void group_copy(uint8_t* o, const uint8_t* litb, uint32_t raw, bool in_group, uint32_t ccs_lane,
-                                  uint32_t cb, uint32_t ce, const TokF* stk, uint32_t* sown) {
-  const uint32_t lane = lane_id();
-  for (uint32_t c0 = cb; c0 < ce; c0 += 64) {
-    sown[lane] = 0;
-    wave_fence();
-    const uint32_t jk = ccs_lane <= c0 ? 0u : ccs_lane - c0;
-    if (in_group && jk < 64) atomicMax(&sown[jk], lane + 1);
-    wave_fence();
-    const uint32_t k = min(scan_max(sown[lane]), 64u) - 1u;
-    wave_fence();
-    const uint32_t cj = c0 + lane;
-    const bool live = cj < ce;
-    const uint32_t kk = live ? k : 0u;
-    const TokF tk = stk[kk];
-    const uint32_t opk = tk.opk, L = tk.L, M = tk.M, D = tk.D, lik = tk.lik, cs = tk.cs;
-    const uint32_t w = cj - cs;
-    const uint32_t nlit = (L + 15u) >> 4;
-    const uint32_t ms = opk + L, me = ms + M;
-    const bool litc = w < nlit;
-    const bool pat = !litc && D < 16u;
-    uint32_t rep = 0, within = litc ? w : w - nlit;
-    if (live && !litc && D >= 16u && D < M) divmod_small(within, (D + 15u) >> 4, rep, within);
-    const uint32_t base = litc ? opk : ms + rep * D;
-    const uint32_t y0 = base + 16u * within;
-    const uint32_t cend = litc ? ms : pat ? me : min(me, base + D);
-    const uint32_t len = live ? min(16u, cend - y0) : 0u;
-    const uint32_t s0 = litc ? y0 : pat ? ms - D : y0 - (rep + 1u) * D;
-    const uint32_t need = pat ? D : len;
-    const bool from_out = s0 < opk;
-    const bool slow = live && from_out && s0 + 16u > raw;
-    const bool straddle = live && !slow && from_out && s0 + need > opk;
-    const uint8_t* a1 = from_out ? o + s0 : litb + lik + (s0 - opk);
-    U128 v = ld128(live && !slow ? a1 : litb);
-    if (straddle) v = splice(v, ld128(litb + lik), opk - s0);
-    if (live && !slow && pat) v = pattern16(v, mod_small(16u * within, D), D);
-    if (live && !slow) {
-      if (len == 16u) st128(o + y0, v);
-      else st_partial(o + y0, v, len);
-    }
-    if (slow) {  // exact, byte by byte
-      const uint32_t ph = pat ? mod_small(16u * within, D) : 0u;
-      for (uint32_t i = 0; i < len; ++i) {
-        const uint32_t si = pat ? s0 + mod_small(ph + i, D) : s0 + i;
-        o[y0 + i] = si < opk ? o[si] : litb[lik + (si - opk)];
-      }
-    }
-  }
-}
-
 // ---------------------------------------------------------------- the LZ replay through an LDS window
 // The last RING bytes of a block's output stay in LDS (byte y at ring[y % RING], the first 16 bytes mirrored after
-// the end so that any 16 bytes read at once). The copy chunks of 64 tokens are laid out first: each one's 16 source
-// bytes are requested from global memory together when they are literals or output older than the window, or read
-// from the window when they are recent output; then the groups are written one after another into the window, LDS
-// reads and writes only, and at the end the chunk's output goes to global memory in 16-byte stores. So a chunk of
-// tokens costs one round trip to memory instead of one per group.
+// the end so that any 16 bytes read at once). The copy chunks of 64 tokens are laid out a batch of 192 at a time:
+// each one's 16 source bytes are requested from global memory together when they are literals or output older than
+// the window, or read from the window when they are recent output; then the chunks are written into the window pass
+// by pass, LDS reads and writes only, and the output goes to global memory in 16-byte stores, one chunk of tokens
+// later (or, for a chunk of tokens whose output is longer than the window, a batch later). So a batch costs one round
+// trip to memory instead of one per group.
 constexpr uint32_t RING = 4096;
-constexpr uint32_t RROUNDS = 3;  // copy chunks laid out at once: RROUNDS x 64
+constexpr uint32_t RROUNDS = 3;  // copy chunks laid out at once: RROUNDS x 64 (a batch writes at most 3072 bytes)
 
 __device__ inline U128 ring_read16(const uint8_t* ring, uint32_t y) {
   U128 v;
@@ -580,11 +525,32 @@ __device__ inline void copy_slow(const CDesc& d, uint8_t* ring, const uint8_t* o
   }
 }
 
-// the chunk of tokens' copy chunks [0, C) through the window; lo: the oldest output position the window holds
-__device__ inline void ring_copy(uint8_t* ring, const uint8_t* o, const uint8_t* litb, uint32_t raw, uint32_t lo,
-                                 uint32_t C, bool on, uint32_t ccs_lane, const TokF* stk, uint32_t* sown) {
+// output [a, e) of the block, now in the window, to global memory
+__device__ inline void ring_flush(const uint8_t* ring, uint8_t* o, uint32_t a, uint32_t e) {
   const uint32_t lane = lane_id();
+  const uint32_t A = min((a + 15u) & ~15u, e), E = max(A, e & ~15u);
+  if (a + lane < A) o[a + lane] = ring[(a + lane) & (RING - 1)];
+  for (uint32_t y = A + 16u * lane; y < E; y += 64u * 16u) st128(o + y, ring_read16(ring, y));
+  if (E + lane < e) o[E + lane] = ring[(E + lane) & (RING - 1)];
+}
+
+// The chunk of tokens' copy chunks [0, C), which write its output [op0, op_end), through the window. A batch that
+// writes [ya, yb) (yb <= ya + 3072) reads the window from lo = yb - RING on (a write at y lands on y - RING, before
+// lo), and output before lo comes from global memory: what of it is not there yet, [fl_a, lo), is flushed before the
+// batch is laid out. fl_a: the first output position not yet in global memory (updated).
+__device__ inline void ring_copy(uint8_t* ring, uint8_t* o, const uint8_t* litb, uint32_t raw, uint32_t op0,
+                                 uint32_t op_end, uint32_t C, bool on, uint32_t ccs_lane, const TokF* stk,
+                                 uint32_t* sown, uint32_t& fl_a) {
+  const uint32_t lane = lane_id();
+  uint32_t ya = op0;  // where the batch's output starts
   for (uint32_t base = 0; base < C; base += 64 * RROUNDS) {
+    const uint32_t yb = min(op_end, ya + 16u * 64u * RROUNDS);  // (at least where it ends)
+    const uint32_t lo = yb > RING ? yb - RING : 0u;
+    if (lo > fl_a) {  // (only in a chunk of tokens whose output is longer than the window)
+      ring_flush(ring, o, fl_a, lo);
+      fl_a = lo;
+      wave_fence();
+    }
     CDesc d[RROUNDS];
     U128 pre[RROUNDS];
 #pragma unroll
@@ -638,23 +604,15 @@ __device__ inline void ring_copy(uint8_t* ring, const uint8_t* o, const uint8_t*
     // (every load of this batch has been used; saying so keeps the compiler from waiting at the loop's head, which
     // would also wait for the stores and loads issued before the loop)
     wait_vm_all();
+    ya = readlane(d[RROUNDS - 1].y0 + d[RROUNDS - 1].len, 63);  // (used only when a full batch follows)
   }
-}
-
-// output [a, e) of the block, now in the window, to global memory
-__device__ inline void ring_flush(const uint8_t* ring, uint8_t* o, uint32_t a, uint32_t e) {
-  const uint32_t lane = lane_id();
-  const uint32_t A = min((a + 15u) & ~15u, e), E = max(A, e & ~15u);
-  if (a + lane < A) o[a + lane] = ring[(a + lane) & (RING - 1)];
-  for (uint32_t y = A + 16u * lane; y < E; y += 64u * 16u) st128(o + y, ring_read16(ring, y));
-  if (E + lane < e) o[E + lane] = ring[(E + lane) & (RING - 1)];
 }
 
 // Tokens go 64 at a time: a chunk may hold the end of one group and several whole groups. The fields, positions
 // and offsets of the whole chunk are computed together (scans by DPP), with the next chunk's bytes already
-// requested; then the chunk's groups are written one after another (a match reads output from before its group
-// only, so the tokens of one group copy side by side). Checks follow the sequential decoder
-// (src/codec2.h:173-202), plus the group rule.
+// requested; then its copy chunks go through the window (ring_copy: a match reads output from before its group only,
+// so the tokens of one group copy side by side, and in practice much more goes at once). Checks follow the
+// sequential decoder (src/codec2.h:173-202), plus the group rule.
 __device__ __forceinline__ void dec_lz_block(const Block* __restrict__ blocks, const uint8_t* __restrict__ comp, uint32_t blog,
                                     const uint8_t* __restrict__ lit, const uint8_t* __restrict__ tok,
                                     const uint8_t* __restrict__ ext, const uint8_t* __restrict__ sym,
@@ -694,7 +652,6 @@ __device__ __forceinline__ void dec_lz_block(const Block* __restrict__ blocks, c
   uint32_t t = 0, op = 0, li = 0, ei = 0, bp = 0;
   uint32_t g = 0, grem = 0, gs_open = 0;  // groups started so far; tokens left in the open group; its output start
   uint32_t r0 = 1, r1 = 4, r2 = 8, l0 = 1, l1 = 4, l2 = 8;
-  uint32_t ring_lo = 0;  // output before this is not in the window (it went to global memory another way)
   uint32_t fl_a = 0, fl_e = 0;  // output [fl_a, fl_e) is in the window and not yet in global memory
   uint32_t fail = 0;
   uint32_t tv_n = lane < nt ? tokb[lane] : 0u, sv_n = lane < nt ? symb[lane] : 0u;
@@ -747,7 +704,6 @@ __device__ __forceinline__ void dec_lz_block(const Block* __restrict__ blocks, c
     const uint32_t bq = bp + (((pk >> 10) & 2047u) - nb_);
     const uint32_t rg = (pk >> 21) - (grem == 0 ? 1u : 0u);  // the token's group, from 0 in the chunk
     const uint32_t tail = readlane(pk, 63);
-    const uint32_t ngc = readlane(rg, c - 1) + 1;
     if (on && (uint64_t)xq + nx > ne) bad = true;
     uint32_t L = tv >> 4, M = on ? (tv & 15u) + 4u : 0u;
     // the token's ext bytes (at most 5) and offset bits (at most 19), requested together by every lane at clamped
@@ -839,21 +795,9 @@ __device__ __forceinline__ void dec_lz_block(const Block* __restrict__ blocks, c
     g = g2;
     grem = grem2;
     const uint32_t op0 = op_end - readlane(itot, 63);  // (the chunk's output: [op0, op_end))
-    if (op_end - op0 <= RING - 16u) {
-      const uint32_t lo = max(op_end > RING ? op_end - RING : 0u, ring_lo);
-      ring_copy(ring, o, litb, raw, lo, readlane(cinc, 63), on, ccs, stk, sown);
-      fl_a = op0;  // (to global memory with the next chunk, or after the loop)
-      fl_e = op_end;
-    } else {  // a long chunk: group by group through global memory, and the window starts again after it
-      for (uint32_t q = 0; q < ngc; ++q) {
-        const uint64_t qm = ballot(on && rg == q);
-        if (!qm) continue;
-        const uint32_t ka = (uint32_t)__builtin_ctzll(qm), kb = 64u - (uint32_t)__builtin_clzll(qm);
-        group_copy(o, litb, raw, on && rg == q, ccs, readlane(ccs, ka), readlane(cinc, kb - 1), stk, sown);
-        wave_fence();  // this group's output is read by the next
-      }
-      ring_lo = op_end;
-    }
+    fl_a = op0;
+    ring_copy(ring, o, litb, raw, op0, op_end, readlane(cinc, 63), on, ccs, stk, sown, fl_a);
+    fl_e = op_end;  // ([fl_a, fl_e) to global memory with the next chunk, or after the loop)
     wave_fence();
   }
   if (!fail && (g != ng || grem != 0 || op + (nl - li) != raw)) fail = E_TOKEN;
