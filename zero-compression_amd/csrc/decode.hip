@@ -127,33 +127,29 @@ __device__ inline void rans_step(const uint8_t* slot, const uint32_t* fc, uint8_
   w <<= s8;
   ob += s8;
 }
-// bytes [o, o + 8) of the 16-byte window c0..c3 (o <= 8), big-endian (the first byte in the top 8 bits)
-__device__ inline uint64_t window8(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t o) {
-  const uint32_t q = o >> 2, r = o & 3u;
-  const uint32_t w0 = q == 0 ? c0 : q == 1 ? c1 : c2;
-  const uint32_t w1 = q == 0 ? c1 : q == 1 ? c2 : c3;
-  const uint32_t w2 = q == 0 ? c2 : q == 1 ? c3 : 0u;  // (q = 2 comes with r = 0)
-  const uint32_t lo = __builtin_amdgcn_alignbyte(w1, w0, r), hi = __builtin_amdgcn_alignbyte(w2, w1, r);
-  return (uint64_t)__builtin_bswap32(lo) << 32 | __builtin_bswap32(hi);
+// the LDS ring of a lane's bytes in the stream decoder: LRING bytes (4 units of 16) and a copy of the first 8 after
+constexpr uint32_t LRING = 64;
+// unit u (16 bytes v0..v3) into its ring slot, and slot 0's first 8 bytes also after the ring
+__device__ inline void ring_unit(uint8_t* mr, uint32_t u, uint32_t v0, uint32_t v1, uint32_t v2, uint32_t v3) {
+  const uint32_t v[4] = {v0, v1, v2, v3};
+  __builtin_memcpy(mr + 16u * (u & 3u), v, 16);
+  if ((u & 3u) == 0u) __builtin_memcpy(mr + LRING, v, 8);
 }
-
-// 8 rANS steps of a lane from the 16-byte window c0..c3 (8 steps take at most 16 bytes: 4 from its first 8 bytes,
-// 4 from the 8 after what those took); the symbols go to stg[64 k + lane]. In the last steps of a stream (TAIL), a
-// lane whose symbol index first + 64 k + lane is past n takes no bytes. Returns the bytes taken.
+// 4 rANS steps of a lane, their (at most 8) bytes read from its ring at q, which moves past them; the symbols go to
+// stg[64 (k0 + k) + lane]. In the last steps of a stream (TAIL), a lane whose symbol index first + 64 (k0 + k) +
+// lane is past n takes no bytes.
 template <bool TAIL>
-__device__ inline uint32_t steps8(const uint8_t* slot, const uint32_t* fc, uint8_t* stg, uint32_t c0, uint32_t c1,
-                                  uint32_t c2, uint32_t c3, uint32_t& x, uint32_t first, uint32_t n) {
+__device__ inline void steps4r(const uint8_t* slot, const uint32_t* fc, uint8_t* stg, const uint8_t* mr, uint32_t& q,
+                               uint32_t& x, uint32_t k0, uint32_t first, uint32_t n) {
   const uint32_t lane = lane_id();
+  uint64_t le;
+  __builtin_memcpy(&le, mr + (q & (LRING - 1u)), 8);
+  uint64_t w = (uint64_t)__builtin_bswap32((uint32_t)le) << 32 | __builtin_bswap32((uint32_t)(le >> 32));
   uint32_t ob = 0;
-  uint64_t w = window8(c0, c1, c2, c3, 0u);
 #pragma unroll
   for (uint32_t k = 0; k < 4; ++k)
-    rans_step(slot, fc, stg + k * 64 + lane, !TAIL || first + k * 64 + lane < n, w, x, ob);
-  w = window8(c0, c1, c2, c3, ob >> 3);
-#pragma unroll
-  for (uint32_t k = 4; k < 8; ++k)
-    rans_step(slot, fc, stg + k * 64 + lane, !TAIL || first + k * 64 + lane < n, w, x, ob);
-  return ob >> 3;
+    rans_step(slot, fc, stg + (k0 + k) * 64 + lane, !TAIL || first + (k0 + k) * 64 + lane < n, w, x, ob);
+  q += ob >> 3;
 }
 // the 512 symbols of 8 steps from stg to d, 8 bytes per lane (every lane stores: no branch around the store); the
 // fences keep the LDS writes before and after it in order
@@ -262,70 +258,62 @@ __device__ __forceinline__ void dec_stream(const Block* __restrict__ blocks, con
   }
   __syncthreads();
   // 64 lanes decode side by side: lane j holds symbols j, j+64, ... A step needs 0, 1 or 2 more bytes of the lane
-  // (x >= 2^11 after the step: 2^15 <= x < 2^23 takes one, x < 2^15 two). Steps go 16 at a time over a 32-byte
-  // window of the lane's next bytes (16 steps take at most 32), 8 over its first 16 bytes and 8 over the 16 after
-  // what those took; the 32 bytes after the window are requested when the 16 steps start and used when they end, so
-  // a load has 16 steps to arrive. The symbols collect in LDS and leave 512 at a time in 8-byte stores per lane (the
-  // vector memory counter is shared by loads and stores: a byte store per step would make every wait for a load also
-  // wait for the stores). Reads stop at the payload's end; a lane that consumes more bytes than its length is
-  // damaged.
+  // (x >= 2^11 after the step: 2^15 <= x < 2^23 takes one, x < 2^15 two), so 4 steps take at most 8 and 8 at most
+  // 16. A lane's bytes after its 4-byte state go through a ring of LRING = 64 bytes in LDS, 16 at a time (unit u:
+  // bytes [16u, 16u + 16), in ring slot u % 4; the first 8 bytes of slot 0 are repeated after the ring, so that any 8
+  // bytes read at once): every 4 steps read their 8 bytes at the lane's position with one LDS read. Steps go 8 at a
+  // time. A lane whose ring holds at most 48 bytes past its position requests its next unit as 8 steps start, and
+  // writes it into the ring when the next 8 start (a load has 8 steps to arrive): the ring then always holds the 16
+  // bytes that 8 steps may take, and a unit is written only over bytes already taken. The symbols collect in LDS and
+  // leave 512 at a time in 8-byte stores per lane (the vector memory counter is shared by loads and stores: a byte
+  // store per step would make every wait for a load also wait for the stores). Reads stop at the payload's end; a
+  // lane that consumes more bytes than its length is damaged.
+  __shared__ uint8_t lring[64 * (LRING + 8)];
+  uint8_t* const mr = lring + lane * (LRING + 8);
   const uint8_t* lp = body + data0 + lend - ll;  // this lane's bytes
+  const uint8_t* ub = lp + 4;                      // its unit 0
   const uint32_t lnl = ll;
   const uint8_t* lim = p + blk.comp_len;
   uint32_t x;
   __builtin_memcpy(&x, lp, 4);  // (ll >= 4: inside the lane)
-  uint32_t pos = 4;  // bytes of the lane consumed; the window a0..a7 holds bytes [pos, pos + 32)
-  uint32_t a0, a1, a2, a3, a4, a5, a6, a7, ad0, ad1;
-  load16(lp, pos, lim, a0, a1, a2, a3, ad0);
-  load16(lp, pos + 16, lim, a4, a5, a6, a7, ad1);
-  shift16(a0, a1, a2, a3, ad0);
-  shift16(a4, a5, a6, a7, ad1);
-  wait_vm_all();  // (x and the window are here: the step loop's head then waits for nothing from before the loop)
+#pragma unroll
+  for (uint32_t u = 0; u < 4; ++u) {  // units 0..3
+    uint32_t v0, v1, v2, v3, d;
+    load16(ub, 16u * u, lim, v0, v1, v2, v3, d);
+    if (d) shift16(v0, v1, v2, v3, d);
+    ring_unit(mr, u, v0, v1, v2, v3);
+  }
+  uint32_t U = 4, q = 0;  // units in the ring; bytes taken
+  bool staged = false;    // unit U is on its way in g0..g3 (moved back by gd bytes)
+  uint32_t g0 = 0, g1 = 0, g2 = 0, g3 = 0, gd = 0;
+  wait_vm_all();  // (the step loop's head then waits for nothing from before the loop)
   const uint32_t nsteps = (n + 63) >> 6, nfull = n >> 6;  // (every lane has a symbol in the first nfull steps)
   uint32_t b0 = 0;
-  for (; b0 + 16 <= nfull; b0 += 16) {
-    // the 32 bytes after the window, requested now and used after 16 steps (16 steps take at most 32 bytes)
-    uint32_t p0, p1, p2, p3, p4, p5, p6, p7, pd0, pd1;
-    load16(lp, pos + 32, lim, p0, p1, p2, p3, pd0);
-    load16(lp, pos + 48, lim, p4, p5, p6, p7, pd1);
-    const uint32_t o1 = steps8<false>(slot, fc, stg, a0, a1, a2, a3, x, 0u, 0u);
-    uint32_t c0, c1, c2, c3;  // bytes [o1, o1 + 16) of the window (o1 <= 16)
-    realign16(a0, a1, a2, a3, a4, a5, a6, a7, o1, c0, c1, c2, c3);
-    store512(dest + b0 * 64, stg);  // (after the loads: a wait for them does not wait for this store)
-    const uint32_t o2 = steps8<false>(slot, fc, stg, c0, c1, c2, c3, x, 0u, 0u);
-    // the window moves on by o1 + o2 (<= 32) bytes: bytes [d, d + 32) of a0..a7 then p0..p7 (which arrived
-    // meanwhile; their loads were moved back by pd0 and pd1 bytes only near the payload's end)
-    if (ballot((pd0 | pd1) != 0u)) {
-      shift16(p0, p1, p2, p3, pd0);
-      shift16(p4, p5, p6, p7, pd1);
+  for (; b0 + 8 <= nfull; b0 += 8) {
+    if (staged) {
+      if (gd) shift16(g0, g1, g2, g3, gd);
+      ring_unit(mr, U, g0, g1, g2, g3);
+      ++U;
     }
-    const uint32_t d = o1 + o2;
-    const bool hi = d > 16u;
-    const uint32_t e = hi ? d - 16u : d;
-    realign16(hi ? a4 : a0, hi ? a5 : a1, hi ? a6 : a2, hi ? a7 : a3, hi ? p0 : a4, hi ? p1 : a5, hi ? p2 : a6,
-              hi ? p3 : a7, e, c0, c1, c2, c3);
-    realign16(hi ? p0 : a4, hi ? p1 : a5, hi ? p2 : a6, hi ? p3 : a7, hi ? p4 : p0, hi ? p5 : p1, hi ? p6 : p2,
-              hi ? p7 : p3, e, a4, a5, a6, a7);
-    a0 = c0; a1 = c1; a2 = c2; a3 = c3;
-    pos += d;
-    store512(dest + (b0 + 8) * 64, stg);
+    staged = 16u * U - q <= 48u;
+    if (staged) load16(ub, 16u * U, lim, g0, g1, g2, g3, gd);
+    steps4r<false>(slot, fc, stg, mr, q, x, 0u, 0u, 0u);
+    steps4r<false>(slot, fc, stg, mr, q, x, 4u, 0u, 0u);
+    store512(dest + b0 * 64, stg);  // (after the load: a wait for it does not wait for this store)
   }
-  if (b0 < nsteps) {  // the last (at most 16) steps, within the window
-    const uint32_t o1 = steps8<true>(slot, fc, stg, a0, a1, a2, a3, x, b0 * 64, n);
-    pos += o1;
-    wave_fence();
-    wave_copy_lds(dest + b0 * 64, stg, min(n - b0 * 64, 512u));
-    wave_fence();
-    if (b0 + 8 < nsteps) {
-      uint32_t c0, c1, c2, c3;
-      realign16(a0, a1, a2, a3, a4, a5, a6, a7, o1, c0, c1, c2, c3);
-      pos += steps8<true>(slot, fc, stg, c0, c1, c2, c3, x, (b0 + 8) * 64, n);
-      wave_fence();
-      wave_copy_lds(dest + (b0 + 8) * 64, stg, n - (b0 + 8) * 64);
-      wave_fence();
-    }
+  if (staged) {
+    if (gd) shift16(g0, g1, g2, g3, gd);
+    ring_unit(mr, U, g0, g1, g2, g3);
+    ++U;
   }
-  if (ballot(pos > lnl) && lane == 0) atomicOr(&err[b], E_LANE);
+  if (b0 < nsteps) {  // the last (at most 8) steps, within the ring
+    steps4r<true>(slot, fc, stg, mr, q, x, 0u, b0 * 64, n);
+    steps4r<true>(slot, fc, stg, mr, q, x, 4u, b0 * 64, n);
+    wave_fence();
+    wave_copy_lds(dest + b0 * 64, stg, n - b0 * 64);
+    wave_fence();
+  }
+  if (ballot(q + 4u > lnl) && lane == 0) atomicOr(&err[b], E_LANE);
 }
 // the literal streams of every block first, then the other four streams: a literal stream is usually many times
 // longer than the others, and started first its wavefronts do not trail the kernel's end
