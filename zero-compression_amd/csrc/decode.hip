@@ -59,21 +59,6 @@ __device__ inline Sections read_sections(const uint8_t* p, uint32_t clen, uint32
   return S;
 }
 
-struct U128W {
-  uint64_t lo, hi;
-};
-// a wavefront copies n bytes from LDS to d (16 bytes per lane per step)
-__device__ inline void wave_copy_lds(uint8_t* d, const uint8_t* s, uint32_t n) {
-  const uint32_t lane = lane_id();
-  const uint32_t full = n & ~15u;
-  for (uint32_t i = lane * 16; i < full; i += 64 * 16) {
-    U128W v;
-    __builtin_memcpy(&v, s + i, 16);
-    __builtin_memcpy(d + i, &v, 16);
-  }
-  if (full + lane < n) d[full + lane] = s[full + lane];
-}
-
 // the 16 bytes of a lane at lp + at, requested without a branch: the load starts at min(lp + at, lim - 16) (lim - 16
 // is inside the payload, which is at least 20 bytes), and d (0..16) is how far the wanted bytes lie past that start;
 // shift16 then drops those d bytes and fills zeros, for the bytes past lim
@@ -109,16 +94,15 @@ __device__ inline void shift16(uint32_t& v0, uint32_t& v1, uint32_t& v2, uint32_
   realign16(v0, v1, v2, v3, 0u, 0u, 0u, 0u, d, v0, v1, v2, v3);
 }
 
-// one rANS decoding step of a lane: the symbol of state x (slot table, then its freq | cum << 16) goes to *out. The
+// one rANS decoding step of a lane: returns the symbol of state x (slot table, then its freq | cum << 16). The
 // lane's next bytes are w, big-endian (the first byte in the top 8 bits); the renormalisation shifts 0, 1 or 2 of
 // them into the state: two below 2^15, one below 2^23, read off the leading zeros of the new state (those of
 // xs | 2^11, at most 20, so that the count is 2 for every state below 2^15). ob counts the bits taken.
-__device__ inline void rans_step(const uint8_t* slot, const uint32_t* fc, uint8_t* out, bool valid, uint64_t& w,
-                                 uint32_t& x, uint32_t& ob) {
+__device__ inline uint32_t rans_step(const uint8_t* slot, const uint32_t* fc, bool valid, uint64_t& w, uint32_t& x,
+                                     uint32_t& ob) {
   const uint32_t sl = x & 4095u;
   const uint32_t sy = slot[sl];
   const uint32_t e = fc[sy];
-  *out = (uint8_t)sy;
   const uint32_t xs = (e & 0xFFFFu) * (x >> 12) + sl - (e >> 16);
   const uint32_t lz = (uint32_t)__builtin_clz(xs | 0x800u);  // 0..20
   const uint32_t s8 = valid ? __builtin_elementwise_sub_sat(lz, 1u) & 24u : 0u;
@@ -126,6 +110,7 @@ __device__ inline void rans_step(const uint8_t* slot, const uint32_t* fc, uint8_
   x = (uint32_t)(xw >> 32);
   w <<= s8;
   ob += s8;
+  return sy;
 }
 // the LDS ring of a lane's bytes in the stream decoder: LRING bytes (4 units of 16) and a copy of the first 8 after
 constexpr uint32_t LRING = 64;
@@ -135,11 +120,12 @@ __device__ inline void ring_unit(uint8_t* mr, uint32_t u, uint32_t v0, uint32_t 
   __builtin_memcpy(mr + 16u * (u & 3u), v, 16);
   if ((u & 3u) == 0u) __builtin_memcpy(mr + LRING, v, 8);
 }
-// 4 rANS steps of a lane, their (at most 8) bytes read from its ring at q, which moves past them; the symbols go to
-// stg[64 (k0 + k) + lane]. In the last steps of a stream (TAIL), a lane whose symbol index first + 64 (k0 + k) +
-// lane is past n takes no bytes.
+// 4 rANS steps of a lane, their (at most 8) bytes read from its ring at q, which moves past them; symbol 64 (k0 + k)
+// + lane of the 8 steps from d goes to d[64 (k0 + k) + lane]: the 64 lanes of a step store 64 consecutive bytes. In
+// the last steps of a stream (TAIL), a lane whose symbol index first + 64 (k0 + k) + lane is past n takes no bytes
+// and stores nothing.
 template <bool TAIL>
-__device__ inline void steps4r(const uint8_t* slot, const uint32_t* fc, uint8_t* stg, const uint8_t* mr, uint32_t& q,
+__device__ inline void steps4r(const uint8_t* slot, const uint32_t* fc, uint8_t* d, const uint8_t* mr, uint32_t& q,
                                uint32_t& x, uint32_t k0, uint32_t first, uint32_t n) {
   const uint32_t lane = lane_id();
   uint64_t le;
@@ -147,19 +133,12 @@ __device__ inline void steps4r(const uint8_t* slot, const uint32_t* fc, uint8_t*
   uint64_t w = (uint64_t)__builtin_bswap32((uint32_t)le) << 32 | __builtin_bswap32((uint32_t)(le >> 32));
   uint32_t ob = 0;
 #pragma unroll
-  for (uint32_t k = 0; k < 4; ++k)
-    rans_step(slot, fc, stg + (k0 + k) * 64 + lane, !TAIL || first + (k0 + k) * 64 + lane < n, w, x, ob);
+  for (uint32_t k = 0; k < 4; ++k) {
+    const bool valid = !TAIL || first + (k0 + k) * 64 + lane < n;
+    const uint32_t sy = rans_step(slot, fc, valid, w, x, ob);
+    if (!TAIL || valid) d[(k0 + k) * 64 + lane] = (uint8_t)sy;
+  }
   q += ob >> 3;
-}
-// the 512 symbols of 8 steps from stg to d, 8 bytes per lane (every lane stores: no branch around the store); the
-// fences keep the LDS writes before and after it in order
-__device__ inline void store512(uint8_t* d, const uint8_t* stg) {
-  const uint32_t lane = lane_id();
-  wave_fence();
-  uint64_t v;
-  __builtin_memcpy(&v, stg + lane * 8, 8);
-  __builtin_memcpy(d + lane * 8, &v, 8);
-  wave_fence();
 }
 
 // ---------------------------------------------------------------- the streams: one wavefront per (block, stream)
@@ -197,7 +176,6 @@ __device__ __forceinline__ void dec_stream(const Block* __restrict__ blocks, con
   __shared__ uint32_t fc[256];       // freq | cum << 16
   __shared__ uint32_t slot32[1024];  // slot -> symbol (bytes, filled four at a time)
   uint8_t* const slot = (uint8_t*)slot32;
-  __shared__ uint8_t stg[512];       // decoded symbols on their way out
 
   // frequencies of the present symbols, in symbol order; lane k owns symbols 4k..4k+3
   uint32_t np = lane < 32 ? (uint32_t)__popc(body[lane]) : 0u;
@@ -264,10 +242,10 @@ __device__ __forceinline__ void dec_stream(const Block* __restrict__ blocks, con
   // bytes read at once): every 4 steps read their 8 bytes at the lane's position with one LDS read. Steps go 8 at a
   // time. A lane whose ring holds at most 48 bytes past its position requests its next unit as 8 steps start, and
   // writes it into the ring when the next 8 start (a load has 8 steps to arrive): the ring then always holds the 16
-  // bytes that 8 steps may take, and a unit is written only over bytes already taken. The symbols collect in LDS and
-  // leave 512 at a time in 8-byte stores per lane (the vector memory counter is shared by loads and stores: a byte
-  // store per step would make every wait for a load also wait for the stores). Reads stop at the payload's end; a
-  // lane that consumes more bytes than its length is damaged.
+  // bytes that 8 steps may take, and a unit is written only over bytes already taken. Each step's 64 symbols leave in
+  // one store of 64 consecutive bytes (a lane's unit load is waited for 8 steps later, past the stores issued since,
+  // which it does not wait for). Reads stop at the payload's end; a lane that consumes more bytes than its length is
+  // damaged.
   __shared__ uint8_t lring[64 * (LRING + 8)];
   uint8_t* const mr = lring + lane * (LRING + 8);
   const uint8_t* lp = body + data0 + lend - ll;  // this lane's bytes
@@ -297,9 +275,8 @@ __device__ __forceinline__ void dec_stream(const Block* __restrict__ blocks, con
     }
     staged = 16u * U - q <= 48u;
     if (staged) load16(ub, 16u * U, lim, g0, g1, g2, g3, gd);
-    steps4r<false>(slot, fc, stg, mr, q, x, 0u, 0u, 0u);
-    steps4r<false>(slot, fc, stg, mr, q, x, 4u, 0u, 0u);
-    store512(dest + b0 * 64, stg);  // (after the load: a wait for it does not wait for this store)
+    steps4r<false>(slot, fc, dest + b0 * 64, mr, q, x, 0u, 0u, 0u);
+    steps4r<false>(slot, fc, dest + b0 * 64, mr, q, x, 4u, 0u, 0u);
   }
   if (staged) {
     if (gd) shift16(g0, g1, g2, g3, gd);
@@ -307,11 +284,8 @@ __device__ __forceinline__ void dec_stream(const Block* __restrict__ blocks, con
     ++U;
   }
   if (b0 < nsteps) {  // the last (at most 8) steps, within the ring
-    steps4r<true>(slot, fc, stg, mr, q, x, 0u, b0 * 64, n);
-    steps4r<true>(slot, fc, stg, mr, q, x, 4u, b0 * 64, n);
-    wave_fence();
-    wave_copy_lds(dest + b0 * 64, stg, n - b0 * 64);
-    wave_fence();
+    steps4r<true>(slot, fc, dest + b0 * 64, mr, q, x, 0u, b0 * 64, n);
+    steps4r<true>(slot, fc, dest + b0 * 64, mr, q, x, 4u, b0 * 64, n);
   }
   if (ballot(q + 4u > lnl) && lane == 0) atomicOr(&err[b], E_LANE);
 }
