@@ -236,6 +236,40 @@ def test_stored_and_method_errors(gpu):
         gpu.decompress([(2, 10, b"0123456789")], 16)
 
 
+def _hash5(v, w):
+    """the 5-byte hash of kernels/zero.cl:49-58 (v: the first 4 bytes little endian, w: the fifth)"""
+    return ((v * 2654435761) ^ (w * 0x9E3779B1 + (w >> 3))) & 0xFFFFFFFF
+
+
+@pytest.mark.parametrize("bl", [16, 18])
+def test_encode_large_buckets(gpu, bl):
+    """buckets of more than 4096 positions: a zero run with a few other keys in its bucket (sorted through a bitmap of
+    positions) and with many (a 2-byte pattern whose windows 'ababa' fall in the zero run's bucket under another
+    tag: the radix sort); payloads equal the oracle's model at levels 1, 5 and 9"""
+    hb = bl - 3
+    hz = _hash5(0, 0)
+    hp = _hash5(16 | 246 << 8 | 16 << 16 | 246 << 24, 16)
+    assert hz >> (32 - hb) == hp >> (32 - hb)  # one bucket
+    assert (hz >> (20 - hb)) & 0xFFF != (hp >> (20 - hb)) & 0xFFF  # two tags
+    B = 1 << bl
+    rng = np.random.default_rng(bl)
+    pat = np.tile(np.array([16, 246], np.uint8), 6000)
+    blocks = []
+    for few in (True, False):
+        d = rng.integers(0, 256, B, dtype=np.uint8)
+        d[1000:1000 + 24000] = 0
+        d[30000:30000 + (600 if few else pat.size)] = pat[: 600 if few else pat.size]
+        blocks.append(d.tobytes())
+    data = b"".join(blocks)
+    for level in (1, 5, 9):
+        got = gpu.compress(data, bl, level)
+        assert len(got) == len(blocks)
+        for k, (blk, (method, payload)) in enumerate(zip(blocks, got)):
+            want, st = O.gpu_compress_block(blk, bl, level)
+            assert st == 0 and method == 1 and payload == want, (level, k)
+        assert gpu.decompress([(m, len(b), p) for b, (m, p) in zip(blocks, got)], bl) == data
+
+
 def test_encode_past_one_launch(gpu):
     """more than 2^31 bytes in one zgpu_encode_blocks call (8193 blocks of 2^18) runs as several launches and gives
     every block the payload it gets alone: the blocks are 128 copies of 64 blocks, so each must match its copy's

@@ -300,6 +300,8 @@ int zgpu_open(int ordinal, zgpu_t** out) {
                             160 * 1024);
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_enc_place), hipFuncAttributeMaxDynamicSharedMemorySize,
                             160 * 1024);
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_enc_sort_l), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)sort_l_lds_bytes(20));
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_enc_parse), hipFuncAttributeMaxDynamicSharedMemorySize,
                             160 * 1024);
   if (const char* pe = getenv("ZERO_PROFILE")) g->profile = atoi(pe) != 0;
@@ -442,14 +444,15 @@ static int encode_run(zgpu_t* g, const uint8_t* d_in, const Block* blocks, uint3
   const uint32_t posgroups = nb << (blog - 8);
   if (prof) stage_begin(g, st);
   k_fill<<<1, 64, 0, st>>>((uint8_t*)q.counts, 0, 16);
-  const uint32_t pbits = pbits_of(blog, S.hbits), NP = 1u << pbits;
-  k_enc_hash<<<nb, 1024, (HB + 16 + 17 * NP) * 4, st>>>(blocks, d_in, blog, S.hbits, pbits, mt, bstart);
+  const uint32_t pbits = pbits_of(blog, S.hbits);
+  k_enc_hash<<<nb, 1024, hash_lds_words(S.hbits, pbits) * 4, st>>>(blocks, d_in, blog, S.hbits, pbits, mt, bstart);
   k_enc_place<<<nb << pbits, 1024, ((1u << (S.hbits - pbits)) + PLACE_CAP) * 4, st>>>(blocks, d_in, nb, blog, S.hbits,
                                                                                      S.keybits, pbits, mt, bstart, srt);
   if (prof) stage(g, st, "hash");
   k_enc_sort_s<<<(unsigned)(((size_t)nb * HB + 255) / 256), 256, 0, st>>>(blocks, nb, blog, S.hbits, srt, bstart, q);
   k_enc_sort_m<<<std::min<uint32_t>(2048, L.qcap), 256, 0, st>>>(blog, S.hbits, srt, bstart, q);
-  k_enc_sort_l<<<std::min<uint32_t>(1024, L.qcap), 1024, 0, st>>>(blog, S.hbits, srt, mt, bstart, q);
+  k_enc_sort_l<<<std::min<uint32_t>(1024, L.qcap), 1024, sort_l_lds_bytes(blog), st>>>(blog, S.hbits, srt, mt, bstart,
+                                                                                      q);
   if (prof) stage(g, st, "sort");
   // the match array cleared: the match kernel writes only the positions that get a match
   k_zero16<<<(unsigned)std::min<size_t>(((size_t)nb << blog) / 1024 + 1, 16384), 256, 0, st>>>((uint4*)mt,

@@ -103,33 +103,66 @@ __host__ __device__ inline uint32_t pbits_of(uint32_t blog, uint32_t hbits) {
   const uint32_t want = blog > 13u ? blog - 13u : 0u;
   return want < hbits ? want : hbits;
 }
+// LDS counters that many lanes of a wavefront may hit at once: the positions of a long run of one byte value all hash
+// alike, and atomics of many lanes on one address are served one after another. crowded(): whether more than 16 of
+// the active lanes share the index of the first of them (a hint, taken once for several atomics); lds_inc(.., true)
+// then gives the lanes where `on` that share the first one's index one atomic between them, ranked by lane. Either
+// form returns each lane's old value of a[i] (after an atomicAdd of 1).
+__device__ inline bool crowded(uint32_t i) {
+  return popc64(ballot(i == (uint32_t)__builtin_amdgcn_readfirstlane((int)i))) > 16u;
+}
+__device__ inline uint32_t lds_inc(uint32_t* a, uint32_t i, bool on, bool crowd) {
+  if (!crowd) return on ? atomicAdd(&a[i], 1u) : 0u;  // (uniform)
+  const uint64_t act = ballot(on);
+  if (!act) return 0u;
+  const uint32_t lead = (uint32_t)__builtin_ctzll(act);
+  const uint32_t first = readlane(i, lead);
+  const bool same = on && i == first;
+  const uint64_t sm = ballot(same);
+  uint32_t b = 0;
+  if (lane_id() == lead) b = atomicAdd(&a[first], popc64(sm));
+  b = readlane(b, lead);
+  if (same) return b + mbcnt64(sm);
+  return on ? atomicAdd(&a[i], 1u) : 0u;
+}
+// k_enc_hash's LDS: the bucket counters (at least 4096 words: later the 16 wavefronts' staging), 16 wavefront sums,
+// a flag, the partition cursors, and per wavefront its partition counts and bases
+constexpr uint32_t HASH_STAGE = 256;  // positions a wavefront stages per step
+__host__ __device__ inline uint32_t hash_counter_words(uint32_t hbits) {
+  return (1u << hbits) > 16u * HASH_STAGE ? 1u << hbits : 16u * HASH_STAGE;
+}
+__host__ __device__ inline size_t hash_lds_words(uint32_t hbits, uint32_t pbits) {
+  return (size_t)hash_counter_words(hbits) + 17 + 33 * ((size_t)1 << pbits);
+}
 __device__ __forceinline__ void enc_hash_block(const Block* __restrict__ blocks, const uint8_t* __restrict__ in, uint32_t blog,
                                       uint32_t hbits, uint32_t pbits, uint32_t* __restrict__ tmp,
                                       uint32_t* __restrict__ bstart, uint32_t b) {
-  extern __shared__ uint32_t cnt[];  // HB counters, 16 wavefront sums, the partition cursors, the claims
+  extern __shared__ uint32_t cnt[];  // hash_lds_words(hbits, pbits)
   const uint32_t tid = threadIdx.x;
-  const uint32_t HBITS = hbits, HB = 1u << HBITS;
+  const uint32_t HBITS = hbits, HB = 1u << HBITS, CW = hash_counter_words(hbits);
   const Block blk = blocks[b];
   const uint8_t* inb = in + blk.src;
   const uint32_t n = enc_len(blk, blog);
   const uint32_t count = n >= 4 ? n - 3 : 0;
+  uint32_t* bigb = cnt + CW + 16;  // (after the 16 wavefront sums) a bucket of more than 4096 entries exists
   for (uint32_t i = tid; i < HB; i += 1024) cnt[i] = 0;
+  if (tid == 0) *bigb = 0u;
   __syncthreads();
   // 4 positions per lane from one 8-byte load, the next load in flight while these are counted
   {
     uint64_t x = n >= 8 ? ld8z(inb, n, 4 * tid) : ld8z_small(inb, n, 4 * tid);
     for (uint32_t q = 4 * tid; q < count; q += 4096) {
       const uint64_t nx = n >= 8 ? ld8z(inb, n, q + 4096) : 0ull;
+      const bool crowd = crowded(hash5_of(x, 0) >> (32 - HBITS));
 #pragma unroll
-      for (uint32_t j = 0; j < 4; ++j)
-        if (q + j < count) atomicAdd(&cnt[hash5_of(x, j) >> (32 - HBITS)], 1u);
+      for (uint32_t j = 0; j < 4; ++j) lds_inc(cnt, hash5_of(x, j) >> (32 - HBITS), q + j < count, crowd);
       x = nx;
     }
   }
   __syncthreads();
   // exclusive scan of the HB counters, HB/16 per wavefront, 64 consecutive counters per step (no LDS bank conflicts)
   const uint32_t wv = tid >> 6, lane = tid & 63u, chunk = HB / 16u;
-  uint32_t* wsum = cnt + HB;  // [16]
+  uint32_t* wsum = cnt + CW;  // [16]
   {
     uint32_t s = 0;
     for (uint32_t i = lane; i < chunk; i += 64) s += cnt[wv * chunk + i];
@@ -147,42 +180,68 @@ __device__ __forceinline__ void enc_hash_block(const Block* __restrict__ blocks,
       const uint32_t c = in ? cnt[i] : 0u;
       const uint32_t inc = wave_scan_incl(c);
       if (in) { cnt[i] = run + inc - c; bs[i] = run + inc - c; }
+      if (in && c > 4096u) *bigb = 1u;  // (lanes that write write the same value)
       run += __builtin_amdgcn_readlane(inc, 63);
     }
   }
   if (tid == 0) bs[HB] = count;
   __syncthreads();
-  // the positions grouped by partition (in no set order inside one: k_enc_place does not need it): a wavefront counts
-  // its 256 positions per partition in LDS counters of its own (the atomics' return values rank the lanes), claims
-  // each partition's count once from the shared cursors, and the lanes place their positions after the claims. No
+  const bool big = *bigb != 0u;  // a bucket of more than 4096: a long run, whose positions crowd the counters
+  // the positions grouped by partition (in no set order inside one: k_enc_place does not need it). A wavefront takes
+  // 256 positions a step: LDS counters of its own rank them per partition (the atomics' return values), one claim
+  // per partition from the shared cursors places each partition's run, and the positions, put in partition order in
+  // the wavefront's staging, go out as runs of consecutive slots (a store instruction covers a few runs, instead of
+  // about one slot in every partition: each slot written at a scattered place costs an L2 request of its own). No
   // workgroup barrier inside the loop.
-  const uint32_t NP = 1u << pbits;
-  uint32_t* pcur = cnt + HB + 16;       // [NP] next free slot of each partition
-  uint32_t* wc = pcur + NP + wv * NP;   // [16][NP] this wavefront's counts, then its claims
+  const uint32_t NP = 1u << pbits, pmask = (1u << blog) - 1u;
+  uint32_t* pcur = cnt + CW + 17;            // [NP] next free slot of each partition
+  uint32_t* wc = pcur + NP + wv * 2u * NP;   // [16][2 NP]: per partition this step's count, then its offset among
+                                             // the step's positions; [NP + i]: its first slot minus that offset
+  uint32_t* wst = cnt + wv * HASH_STAGE;     // [16][HASH_STAGE] (the counters' words, no longer needed)
   if (tid < NP) pcur[tid] = cnt[tid << (HBITS - pbits)];
   __syncthreads();
   uint32_t* tb = tmp + ((size_t)b << blog);
   uint64_t x = n >= 8 ? ld8z(inb, n, 4 * tid) : ld8z_small(inb, n, 4 * tid);
-  for (uint32_t q = 4 * tid; q - 4 * lane < count; q += 4096) {  // (a wavefront runs while any lane has work)
+  for (uint32_t q = 4 * tid; q - 4 * lane < count; q += 4096) {  // (uniform: a wavefront runs while any lane has work)
     const uint64_t nx = n >= 8 ? ld8z(inb, n, q + 4096) : 0ull;
     for (uint32_t i = lane; i < NP; i += 64) wc[i] = 0;
     wave_fence();
     uint32_t d[4], rank[4];
 #pragma unroll
-    for (uint32_t j = 0; j < 4; ++j) {
-      d[j] = pbits ? hash5_of(x, j) >> (32 - pbits) : 0u;
-      rank[j] = q + j < count ? atomicAdd(&wc[d[j]], 1u) : 0u;
-    }
-    wave_fence();
-    for (uint32_t i = lane; i < NP; i += 64) {
-      const uint32_t c = wc[i];
-      wc[i] = c ? atomicAdd(&pcur[i], c) : 0u;
-    }
-    wave_fence();
+    for (uint32_t j = 0; j < 4; ++j) d[j] = pbits ? hash5_of(x, j) >> (32 - pbits) : 0u;
+    const bool crowd = big && crowded(d[0]);
 #pragma unroll
-    for (uint32_t j = 0; j < 4; ++j)
-      if (q + j < count) tb[wc[d[j]] + rank[j]] = q + j;
-    wave_fence();  // (the next step's counts overwrite wc)
+    for (uint32_t j = 0; j < 4; ++j) rank[j] = lds_inc(wc, d[j], q + j < count, crowd);
+    wave_fence();
+    uint32_t run = 0;  // (after the loop: the step's positions)
+    for (uint32_t i0 = 0; i0 < NP; i0 += 64) {
+      const uint32_t i = i0 + lane;
+      const uint32_t c = i < NP ? wc[i] : 0u;
+      const uint32_t inc = wave_scan_incl(c);
+      if (i < NP) {
+        const uint32_t off = run + inc - c;
+        const uint32_t first = c ? atomicAdd(&pcur[i], c) : 0u;
+        wc[i] = off;
+        wc[NP + i] = first - off;
+      }
+      run += readlane(inc, 63);
+    }
+    wave_fence();
+    if (pbits >= 4) {
+#pragma unroll
+      for (uint32_t j = 0; j < 4; ++j)
+        if (q + j < count) wst[wc[d[j]] + rank[j]] = d[j] << blog | (q + j);
+      wave_fence();
+      for (uint32_t L = lane; L < run; L += 64) {
+        const uint32_t v = wst[L];
+        tb[wc[NP + (v >> blog)] + L] = v & pmask;
+      }
+    } else {  // (fewer than 16 partitions: the runs are long enough as the lanes hold them)
+#pragma unroll
+      for (uint32_t j = 0; j < 4; ++j)
+        if (q + j < count) tb[wc[NP + d[j]] + wc[d[j]] + rank[j]] = q + j;
+    }
+    wave_fence();  // (the next step's counts overwrite wc, its positions wst)
     x = nx;
   }
 }
@@ -220,11 +279,13 @@ __device__ __forceinline__ void enc_place_part(const Block* __restrict__ blocks,
     for (uint32_t u = 0; u < 4; ++u) p[u] = tb[min(i0 + 1024 * u, m - 1)];
 #pragma unroll
     for (uint32_t u = 0; u < 4; ++u) x[u] = n >= 8 ? ld8z(inb, n, p[u]) : ld8z_small(inb, n, p[u]);
+    // (a partition too large for LDS holds a long run's bucket)
+    const bool crowd = !in_lds && crowded((hash5_of(x[0], 0) >> (32 - hbits)) & (NB - 1u));
 #pragma unroll
     for (uint32_t u = 0; u < 4; ++u)
       if (i0 + 1024 * u < m) {
         const uint32_t h = hash5_of(x[u], 0);
-        const uint32_t slot = atomicAdd(&lc[(h >> (32 - hbits)) & (NB - 1u)], 1u);
+        const uint32_t slot = lds_inc(lc, (h >> (32 - hbits)) & (NB - 1u), true, crowd);
         const uint32_t e = ((h >> (32 - keybits)) & tagmask) << blog | p[u];
         if (in_lds) buf[slot] = e;
         else sb[slot] = e;
@@ -256,11 +317,17 @@ struct SortQueues {
 // wavefront sorts them one after another, one entry per lane, a bitonic network over lane shuffles. Larger buckets
 // are queued for k_enc_sort_m (<= 4096) or k_enc_sort_l.
 // the k <= 16 values at s sorted (one thread): Batcher's odd-even merge sort of 16, 63 comparators, each a min and a
-// max (the padding sorts last). All 16 reads are made: past the bucket they read entries that are not written.
+// max (the padding sorts last). The lanes of a wavefront sort 64 buckets at scattered places, so each access costs
+// an L2 request whatever its width up to 16 bytes: the bucket is read 16 bytes at a time (the last read may take up
+// to 3 entries past it, which are not written), and written back 16 bytes at a time where 4 entries remain.
 __device__ inline void sort16(uint32_t* s, uint32_t k) {
   uint32_t v[16];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) v[i] = s[i];
+  for (int q = 0; q < 16; q += 4) {
+    uint32_t t[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+    if ((uint32_t)q < k) __builtin_memcpy(t, s + q, 16);
+    v[q] = t[0]; v[q + 1] = t[1]; v[q + 2] = t[2]; v[q + 3] = t[3];
+  }
 #pragma unroll
   for (int i = 0; i < 16; ++i) v[i] = (uint32_t)i < k ? v[i] : 0xFFFFFFFFu;
 #pragma unroll
@@ -277,8 +344,16 @@ __device__ inline void sort16(uint32_t* s, uint32_t k) {
             v[i + j + kk] = max(a, c);
           }
 #pragma unroll
-  for (int i = 0; i < 16; ++i)
-    if ((uint32_t)i < k) s[i] = v[i];
+  for (int q = 0; q < 16; q += 4) {
+    if ((uint32_t)q + 4u <= k) {
+      const uint32_t t[4] = {v[q], v[q + 1], v[q + 2], v[q + 3]};
+      __builtin_memcpy(s + q, t, 16);
+    } else if ((uint32_t)q < k) {
+      s[q] = v[q];
+      if ((uint32_t)q + 1u < k) s[q + 1] = v[q + 1];
+      if ((uint32_t)q + 2u < k) s[q + 2] = v[q + 2];
+    }
+  }
 }
 // the value of lane ^ j (j = 1, 2, ..., 32) without LDS traffic: DPP inside 16-lane rows (quad permutations, row
 // rotations: row_ror:n gives lane i the value of lane i - n of its row), the gfx950 permlane swaps across rows
@@ -307,6 +382,24 @@ __device__ inline void sort64_wave(uint32_t* s, uint32_t k) {
     }
   if (lane < k) s[lane] = v;
 }
+// two buckets of at most 32 values sorted side by side, the k values at s by lanes 0..31 and the k2 at s2 by lanes
+// 32..63 (k2 = 0: none): the network of 32 (15 steps instead of 21), whose exchanges stay inside each half, every
+// half sorting upwards
+__device__ inline void sort32x2_wave(uint32_t* s, uint32_t k, uint32_t* s2, uint32_t k2) {
+  const uint32_t lane = lane_id(), l = lane & 31u;
+  uint32_t* p = lane < 32u ? s : s2;
+  const uint32_t kk2 = lane < 32u ? k : k2;
+  uint32_t v = l < kk2 ? p[l] : 0xFFFFFFFFu;
+#pragma unroll
+  for (uint32_t kk = 2; kk <= 32; kk <<= 1)
+#pragma unroll
+    for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
+      const uint32_t o = xor_lane(v, j, lane);
+      const bool keep_min = ((l & j) == 0) == ((l & kk) == 0);
+      v = keep_min ? min(v, o) : max(v, o);
+    }
+  if (l < kk2) p[l] = v;
+}
 // the bucket [lo, lo + k) of each thread, at base + (lo - at)
 __device__ inline void sort_small(uint32_t* base, uint32_t at, uint32_t lo, uint32_t k, uint32_t b, uint32_t h,
                                   SortQueues q) {
@@ -317,7 +410,16 @@ __device__ inline void sort_small(uint32_t* base, uint32_t at, uint32_t lo, uint
   } else if (k >= 2 && k <= 16) {
     sort16(base + (lo - at), k);
   }
-  uint64_t m = ballot(k > 16 && k <= 64);
+  uint64_t m = ballot(k > 16 && k <= 32);  // two at a time
+  while (m) {
+    const uint32_t a = (uint32_t)__builtin_ctzll(m);
+    m &= m - 1;
+    const uint32_t c = m ? (uint32_t)__builtin_ctzll(m) : a;
+    m &= m - 1;  // (0 stays 0)
+    sort32x2_wave(base + (readlane(lo, a) - at), readlane(k, a), base + (readlane(lo, c) - at),
+                  c != a ? readlane(k, c) : 0u);
+  }
+  m = ballot(k > 32 && k <= 64);
   while (m) {
     const uint32_t who = (uint32_t)__builtin_ctzll(m);
     m &= m - 1;
@@ -385,16 +487,25 @@ __global__ __launch_bounds__(256) void k_enc_sort_m(uint32_t blog, uint32_t hbit
   enc_sort_m_range(blog, hbits, srt, bstart, q, blockIdx.x, gridDim.x);
 }
 
-// queued buckets of more than 4096 entries: one workgroup of 1024 per bucket, a stable LSD radix sort with 8-bit
-// digits through a temporary of the same size (tmp, indexed like srt), in-wavefront ranks from ballots.
+// queued buckets of more than 4096 entries: one workgroup of 1024 per bucket.
+// Such a bucket is mostly one key: a long run of a byte value, or of any short pattern, gives every position in it the
+// same 5 bytes, and the bucket holds a few other keys besides (collisions of the bucket's hash bits). The entries of
+// the dominant tag are in order once their positions are: they are set in a bitmap of the block's positions (dynamic
+// LDS, sort_l_lds_bytes) and read back 64 at a time, each set bit's lane writing its entry at its rank; the others
+// (at most SORT_L_OTHERS) are ranked among themselves and go before or after that run. Buckets with more others go
+// through a stable LSD radix sort with 8-bit digits through a temporary of the same size (tmp, indexed like srt),
+// in-wavefront ranks from ballots.
+__host__ __device__ inline size_t sort_l_lds_bytes(uint32_t blog) { return ((size_t)1 << blog) / 8; }
+constexpr uint32_t SORT_L_OTHERS = 1024;  // entries of other tags a bucket may hold and still take the bitmap
 // the queued buckets w0, w0 + step, ... of q.large
 __device__ __forceinline__ void enc_sort_l_range(uint32_t blog, uint32_t hbits, uint32_t* __restrict__ srt,
                                         uint32_t* __restrict__ tmp, const uint32_t* __restrict__ bstart, SortQueues q,
                                         uint32_t w0, uint32_t step) {
   __shared__ uint32_t base[256], tile_tot[256];
   __shared__ uint32_t wcnt[16][256];
-  const uint32_t HB = 1u << hbits;
-  const uint32_t tid = threadIdx.x, wv = tid >> 6;
+  extern __shared__ uint32_t bmp[];  // [2^blog / 32]
+  const uint32_t HB = 1u << hbits, B = 1u << blog;
+  const uint32_t tid = threadIdx.x, wv = tid >> 6, lane = tid & 63u;
   const uint32_t total = min(q.counts[1], q.cap);
   for (uint32_t w = w0; w < total; w += step) {
     const uint32_t b = q.large[2 * w], h = q.large[2 * w + 1];
@@ -402,6 +513,57 @@ __device__ __forceinline__ void enc_sort_l_range(uint32_t blog, uint32_t hbits, 
     const uint32_t lo = bs[h], k = bs[h + 1] - lo;
     uint32_t* A = srt + ((size_t)b << blog) + lo;
     uint32_t* T = tmp + ((size_t)b << blog) + lo;
+    // The tag of the middle entry, td: in a bucket of one dominant key, that key's. Its entries go to the bitmap, the
+    // others (collisions of the bucket's hash bits, at most SORT_L_OTHERS of them) to LDS (wcnt's words).
+    const uint32_t td = A[k / 2u] >> blog, tdb = td << blog;
+    uint32_t* oth = &wcnt[0][0];
+    for (uint32_t i = tid; i < B / 32u; i += 1024) bmp[i] = 0;
+    if (tid == 0) { base[0] = 0; base[1] = 0; }
+    __syncthreads();
+    for (uint32_t i0 = 0; i0 < k; i0 += 1024) {  // (uniform)
+      const uint32_t i = i0 + tid;
+      const uint32_t e = i < k ? A[i] : 0u;
+      const bool dom = i < k && (e >> blog) == td, other = i < k && !dom;
+      if (dom) atomicOr(&bmp[(e & (B - 1u)) >> 5], 1u << (e & 31u));
+      const uint64_t om = ballot(other);
+      if (om) {  // (uniform)
+        uint32_t at = 0;
+        if (lane == 0) at = atomicAdd(&base[0], popc64(om));
+        at = readlane(at, 0) + mbcnt64(om);
+        if (other && at < SORT_L_OTHERS) oth[at] = e;
+      }
+    }
+    __syncthreads();
+    const uint32_t m = base[0];
+    if (m <= SORT_L_OTHERS) {  // (uniform)
+      // the others in order: each one's rank among them, from comparisons with all of them (they are distinct)
+      uint32_t ov = 0, orank = 0;
+      if (tid < m) {
+        ov = oth[tid];
+        for (uint32_t j = 0; j < m; ++j) orank += oth[j] < ov ? 1u : 0u;
+        if (ov < tdb) atomicAdd(&base[1], 1u);
+      }
+      // each wavefront's share of the 64-position chunks: its count, then the rank of its first entry
+      const uint32_t per = B / 64u / 16u, c0 = wv * per;
+      uint32_t s = 0;
+      for (uint32_t c = c0 + lane; c < c0 + per; c += 64) s += __popc(bmp[2 * c]) + __popc(bmp[2 * c + 1]);
+      s = wave_sum(s);
+      if (lane == 0) tile_tot[wv] = s;
+      __syncthreads();
+      const uint32_t nlt = base[1];  // others before td's entries
+      if (tid < m) A[orank < nlt ? orank : orank + (k - m)] = ov;
+      uint32_t run = nlt;
+      for (uint32_t x = 0; x < wv; ++x) run += tile_tot[x];
+      for (uint32_t c = c0; c < c0 + per; ++c) {
+        const uint64_t bm = (uint64_t)bmp[2 * c] | (uint64_t)bmp[2 * c + 1] << 32;
+        if (bm) {  // (uniform)
+          if ((bm >> lane) & 1u) A[run + mbcnt64(bm)] = tdb | (64u * c + lane);
+          run += popc64(bm);
+        }
+      }
+      __syncthreads();  // (the next bucket clears the bitmap and reuses base)
+      continue;
+    }
     for (uint32_t i = tid; i < 16 * 256; i += 1024) (&wcnt[0][0])[i] = 0;
     for (uint32_t pass = 0; pass < 4; ++pass) {
       const uint32_t sh = pass * 8;

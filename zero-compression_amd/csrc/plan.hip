@@ -247,8 +247,9 @@ EncShape class_shape(uint32_t blog, uint32_t key_log) {
 }
 size_t parse_lds_bytes(const EncShape& S) { return parse_lds_words(S.blog, S.walkers) * 4; }
 size_t hash_lds_bytes(const EncShape& S) {
-  return (((size_t)1 << S.hbits) + 16 + 17 * ((size_t)1 << pbits_of(S.blog, S.hbits))) * 4;
+  return hash_lds_words(S.hbits, pbits_of(S.blog, S.hbits)) * 4;
 }
+size_t sort_l_lds_shape(const EncShape& S) { return sort_l_lds_bytes(S.blog); }
 size_t place_lds_bytes(const EncShape& S) {
   return (((size_t)1 << (S.hbits - pbits_of(S.blog, S.hbits))) + PLACE_CAP) * 4;
 }
@@ -430,7 +431,7 @@ static int plan_launch(zgpu_plan* p, int op, const uint8_t* d_src, uint8_t* d_ds
       mark();
       k_plan_sort_m<<<plan_grid(p, P_SORT_M), 256, 0, st>>>(t);
       mark();
-      k_plan_sort_l<<<plan_grid(p, P_SORT_L), 1024, 0, st>>>(t);
+      k_plan_sort_l<<<plan_grid(p, P_SORT_L), 1024, pass_lds(p, ps, sort_l_lds_shape), st>>>(t);
       mark();
       k_plan_clear<<<(unsigned)std::max(8, p->g->props.cu_count) * 8, 256, 0, st>>>(t);
       k_plan_match<<<plan_grid(p, P_MATCH), p->match_wg, (p->match_wg / 64) * MATCH_LDS_WORDS * 4, st>>>(
@@ -512,6 +513,8 @@ int zgpu_plan_create(zgpu_t* g, uint32_t nb, const uint64_t* src, const uint32_t
   for (const void* k : {reinterpret_cast<const void*>(&k_plan_hash), reinterpret_cast<const void*>(&k_plan_place),
                         reinterpret_cast<const void*>(&k_plan_parse)})
     HIPCHK(g, hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 158 * 1024));
+  HIPCHK(g, hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plan_sort_l),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)sort_l_lds_bytes(20)));
   zgpu_plan* p = new zgpu_plan();
   p->g = g;
   p->nb = nb;
