@@ -651,16 +651,20 @@ __device__ inline void load_window(const uint8_t* inb, uint32_t n, uint32_t p, u
 }
 
 
-// the number of leading bytes where a and b agree (4 * NW when all do)
+// the number of leading bytes where a and b agree (4 * NW when all do): the first differing bit, 64 bits at a time
+// from the last piece (a 64-bit count of trailing zeros that gives 64 for none is two v_ffbl_b32 and a clamped
+// add); a piece that agrees whole (64) adds the answer of the pieces after it
 template <int NW>
 __device__ inline uint32_t common_prefix(const uint32_t (&a)[NW], const uint32_t (&b)[NW]) {
-  uint32_t l = 4u * NW;
+  static_assert(NW % 2 == 0, "64-bit pieces");
+  uint32_t lb = 0;
 #pragma unroll
-  for (int q = NW - 1; q >= 0; --q) {
-    const uint32_t x = a[q] ^ b[q];
-    if (x) l = 4u * q + ((uint32_t)__builtin_ctz(x) >> 3);
+  for (int q = NW - 2; q >= 0; q -= 2) {
+    const uint64_t x = (uint64_t)(a[q + 1] ^ b[q + 1]) << 32 | (a[q] ^ b[q]);
+    const uint32_t t = (uint32_t)__builtin_ctzg(x, 64);
+    lb = t + (t >> 6) * lb;
   }
-  return l;
+  return lb >> 3;
 }
 
 template <int NW>
@@ -722,13 +726,14 @@ __device__ inline void match_slots(const uint8_t* __restrict__ inb, uint32_t n, 
   }
   wave_fence();
   const uint32_t maxlen = min(lencap, n - p);
-  uint32_t best = 0, bestd = 0;
+  // best starts at 3, so that "longer than best" also means "at least 4 long" (a match is)
+  uint32_t best = 3, bestd = 0;
   if (run && okm) {  // a run of one byte value (kernels/zero.cl:219-222): 5 equal bytes from p - 1
     uint32_t r[NW];
 #pragma unroll
     for (int q = 0; q < NW; ++q) r[q] = w[0];
-    best = min(common_prefix<NW>(w, r), maxlen);
-    bestd = 1;
+    const uint32_t l = min(common_prefix<NW>(w, r), maxlen);
+    if (l > best) { best = l; bestd = 1; }
   }
   // every candidate compared, with selects rather than branches (lanes differ in how many candidates they have)
 #pragma unroll
@@ -740,7 +745,7 @@ __device__ inline void match_slots(const uint8_t* __restrict__ inb, uint32_t n, 
 #pragma unroll
     for (int q = 0; q < NW; ++q) c[q] = lwin[idx * NW + q];
     const uint32_t l = min(common_prefix<NW>(w, c), maxlen);
-    const bool better = ((okm >> (k - 1)) & 1u) && l >= 4 && l > best;
+    const bool better = ((okm >> (k - 1)) & 1u) && l > best;
     best = better ? l : best;
     bestd = better ? p - cp : bestd;
   }
