@@ -400,16 +400,17 @@ __device__ inline void sort32x2_wave(uint32_t* s, uint32_t k, uint32_t* s2, uint
     }
   if (l < kk2) p[l] = v;
 }
-// the bucket [lo, lo + k) of each thread, at base + (lo - at)
-__device__ inline void sort_small(uint32_t* base, uint32_t at, uint32_t lo, uint32_t k, uint32_t b, uint32_t h,
-                                  SortQueues q) {
+// bucket h of block b queued for the larger sorts when it has more than 64 entries
+__device__ inline void queue_large(uint32_t k, uint32_t b, uint32_t h, SortQueues q) {
   if (k > 64) {
     const uint32_t which = k > 4096 ? 1 : 0;
     const uint32_t i = atomicAdd(&q.counts[which], 1u);
     if (i < q.cap) { uint32_t* dst = which ? q.large : q.mid; dst[2 * i] = b; dst[2 * i + 1] = h; }
-  } else if (k >= 2 && k <= 16) {
-    sort16(base + (lo - at), k);
   }
+}
+// the bucket [lo, lo + k) of each thread (k <= 64; larger ones are left as they are), at base + (lo - at)
+__device__ inline void sort_small(uint32_t* base, uint32_t at, uint32_t lo, uint32_t k) {
+  if (k >= 2 && k <= 16) sort16(base + (lo - at), k);
   uint64_t m = ballot(k > 16 && k <= 32);  // two at a time
   while (m) {
     const uint32_t a = (uint32_t)__builtin_ctzll(m);
@@ -428,18 +429,40 @@ __device__ inline void sort_small(uint32_t* base, uint32_t at, uint32_t lo, uint
 }
 
 // bucket h of block b on this thread (valid false: a thread with no bucket, which still takes part in the
-// wavefront's sorts of the others)
+// wavefront's sorts of the others). A wavefront's buckets are consecutive, so their entries are one span of srt: it
+// is read into the wavefront's LDS with coalesced loads, the buckets are sorted there, and it is written back with
+// coalesced stores (sorted in place, the lanes' scattered bucket accesses each cost an L2 request). A span of more
+// than SORT_S_SPAN entries (it holds a bucket queued for the larger sorts), or lanes of more than one block, sort in
+// place. Workgroups of 256 threads.
+constexpr uint32_t SORT_S_SPAN = 1024;
 __device__ __forceinline__ void enc_sort_bucket(uint32_t blog, uint32_t hbits, uint32_t* __restrict__ srt,
                                        const uint32_t* __restrict__ bstart, SortQueues q, uint32_t b, uint32_t h,
                                        bool valid) {
-  const uint32_t HB = 1u << hbits;
+  __shared__ uint32_t span_lds[4][SORT_S_SPAN + 4];  // (+ 4: sort16 reads up to 3 entries past a bucket)
+  const uint32_t HB = 1u << hbits, lane = lane_id();
   uint32_t lo = 0, k = 0;
   if (valid) {
     const uint32_t* bs = bstart + (size_t)b * (HB + 1);
     lo = bs[h];
     k = bs[h + 1] - lo;
   }
-  sort_small(srt + ((size_t)b << blog), 0, lo, k, b, h, q);
+  if (valid) queue_large(k, b, h, q);
+  const uint64_t vm = ballot(valid);
+  if (!vm) return;  // (uniform)
+  const uint32_t first = (uint32_t)__builtin_ctzll(vm), last = 63u - (uint32_t)__builtin_clzll(vm);
+  const uint32_t bf = readlane(b, first);
+  uint32_t* sb = srt + ((size_t)bf << blog);
+  const uint32_t s0 = readlane(lo, first), span = readlane(lo + k, last) - s0;
+  if (ballot(valid && b != bf) || span > SORT_S_SPAN) {  // (uniform)
+    sort_small(srt + ((size_t)b << blog), 0, lo, k);
+    return;
+  }
+  uint32_t* lds = span_lds[threadIdx.x >> 6];
+  for (uint32_t i = lane; i < span; i += 64) lds[i] = sb[s0 + i];
+  wave_fence();
+  sort_small(lds, s0, lo, k);
+  wave_fence();
+  for (uint32_t i = lane; i < span; i += 64) sb[s0 + i] = lds[i];
 }
 __global__ __launch_bounds__(256) void k_enc_sort_s(const Block* __restrict__ blocks, uint32_t nb, uint32_t blog,
                                                     uint32_t hbits, uint32_t* __restrict__ srt,
