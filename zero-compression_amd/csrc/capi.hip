@@ -302,8 +302,10 @@ int zgpu_open(int ordinal, zgpu_t** out) {
                             160 * 1024);
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_enc_sort_l), hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)sort_l_lds_bytes(20));
-  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_enc_parse), hipFuncAttributeMaxDynamicSharedMemorySize,
-                            160 * 1024);
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_enc_parse<true>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_enc_parse<false>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
   if (const char* pe = getenv("ZERO_PROFILE")) g->profile = atoi(pe) != 0;
   *out = g;
   return ZGPU_OK;
@@ -461,7 +463,12 @@ static int encode_run(zgpu_t* g, const uint8_t* d_in, const Block* blocks, uint3
   k_enc_match<<<posgroups, 256, 0, st>>>(blocks, d_in, blog, ncand, lencap, srt, mt);
   if (prof) stage(g, st, "match");
   const size_t parse_lds = parse_lds_words(blog, S.walkers) * 4;
-  k_enc_parse<<<nb, std::max(64u, S.walkers), parse_lds, st>>>(blocks, d_in, blog, S.walkers, lencap, mt, (uint32_t*)(base + L.mark));
+  if (parse_lds_marks(blog))
+    k_enc_parse<true><<<nb, std::max(64u, S.walkers), parse_lds, st>>>(blocks, d_in, blog, S.walkers, lencap, mt,
+                                                                      (uint32_t*)(base + L.mark));
+  else
+    k_enc_parse<false><<<nb, std::max(64u, S.walkers), parse_lds, st>>>(blocks, d_in, blog, S.walkers, lencap, mt,
+                                                                       (uint32_t*)(base + L.mark));
   if (prof) stage(g, st, "parse");
   k_enc_tokens<<<nb, 64, 0, st>>>(blocks, d_in, blog, lencap, mt, (uint32_t*)(base + L.mark), T);
   if (prof) stage(g, st, "tokens");

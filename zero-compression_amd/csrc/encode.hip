@@ -24,10 +24,17 @@ namespace zk {
 
 #define KEY_TAG_BITS 12u
 __host__ __device__ inline uint32_t hbits_of(uint32_t blog) { return blog - 3u < 15u ? blog - 3u : 15u; }
-// the parse's LDS words for a block of 2^blog bytes and `walkers` segments: the bitmap, then per walker its exit,
-// entry, meeting point and a 16-word window
+// the parse's window of matches per walker (entries; 32, a whole 128-byte line, was slower on pages with long
+// matches, whose walks use few entries of a window)
+constexpr uint32_t PARSE_WIN = 16;
+// where the parse gathers a block's chain bitmap: in LDS (copied out at the end) for blocks up to 2^16 bytes, whose
+// bitmap is small beside the rest of the parse's LDS; else in the block's mark words in HBM, each walker storing its
+// own segment's words, so that the bitmap (32 KiB at 2^18, 128 KiB at 2^20) does not limit the workgroups per CU
+__host__ __device__ inline bool parse_lds_marks(uint32_t blog) { return blog <= 16u; }
+// the parse's LDS words for `walkers` segments: the bitmap when it is kept in LDS, then per walker its exit, entry,
+// meeting point and a window (PARSE_WIN + 1 words apart, so that the walkers' window accesses spread over the banks)
 __host__ __device__ inline size_t parse_lds_words(uint32_t blog, uint32_t walkers) {
-  return ((size_t)1 << blog) / 32 + 19 * (size_t)walkers;
+  return (parse_lds_marks(blog) ? ((size_t)1 << blog) / 32 : 0) + (4 + PARSE_WIN) * (size_t)walkers;
 }
 constexpr uint32_t PARSE_SPAN = 1024;       // default parse segment: positions per walker
 constexpr uint32_t PARSE_WALKERS_MIN = 128;  // default walkers per block, at least (each segment still >= 32 positions)
@@ -862,82 +869,113 @@ __device__ inline uint32_t nextpos(uint32_t* m, const uint8_t* inb, uint32_t n, 
   return nextpos_v(m, inb, n, lencap, p, lazy_of(m[p], p + 1 < n ? m[p + 1] : 0u, lencap));
 }
 
-// sets the bits a .. b - 1 of the bitmap bm (a < b <= a + 32)
-__device__ inline void mark_range(uint32_t* bm, uint32_t a, uint32_t b) {
-  const uint64_t bits = (b - a == 64u ? ~0ull : ((1ull << (b - a)) - 1ull)) << (a & 31u);
-  bm[a >> 5] |= (uint32_t)bits;
-  if (((b - 1u) >> 5) != (a >> 5)) bm[(a >> 5) + 1u] |= (uint32_t)(bits >> 32);
-}
+// One thread's marks in a block's chain bitmap, gathered a word at a time in a register and stored as the words are
+// passed, g (1, 2 or 4) aligned words to a store: the thread owns the words it writes (a segment's, a multiple of g
+// words from a multiple of g) and stores each once, in order, zero words included. Fewer stores matter beyond
+// their own cost: a load waits for every memory access issued before it, stores included.
+struct MarkWriter {
+  uint32_t* mk;  // the block's bitmap
+  uint32_t cw;   // the word being gathered
+  uint32_t bits;
+  uint32_t g;
+  uint32_t w0 = 0, w1 = 0, w2 = 0;  // the group's words before cw (the last in w2)
+  __device__ void next() {
+    if (g == 4u && (cw & 3u) == 3u) {
+      *reinterpret_cast<uint4*>(mk + cw - 3u) = make_uint4(w0, w1, w2, bits);
+    } else if (g == 2u && (cw & 1u) == 1u) {
+      *reinterpret_cast<uint2*>(mk + cw - 1u) = make_uint2(w2, bits);
+    } else if (g == 1u) {
+      mk[cw] = bits;
+    }
+    w0 = w1; w1 = w2; w2 = bits;
+    bits = 0;
+    ++cw;
+  }
+  // stores the words before position a's
+  __device__ void to(uint32_t a) {
+    while ((a >> 5) > cw) next();
+  }
+  // sets the bits a .. b - 1 (a < b <= a + 32, a at or past the gathered word's start)
+  __device__ void range(uint32_t a, uint32_t b) {
+    to(a);
+    const uint64_t r = ((1ull << (b - a)) - 1ull) << (a & 31u);
+    bits |= (uint32_t)r;
+    if ((uint32_t)(r >> 32)) { next(); bits = (uint32_t)(r >> 32); }
+  }
+};
 
-// the chain from e through the positions before send: where it meets a mark in bm, that position | 1 << 31; else
-// the first position at or past send
-__device__ inline uint32_t join_walk(uint32_t* m, const uint8_t* inb, uint32_t n, uint32_t lencap, const uint32_t* bm,
+// the chain from e through the positions before send: where it meets a mark in mk, that position | 1 << 31; else
+// the first position at or past send. The mark word and the matches at t are requested together.
+__device__ inline uint32_t join_walk(uint32_t* m, const uint8_t* inb, uint32_t n, uint32_t lencap, const uint32_t* mk,
                                      uint32_t e, uint32_t send) {
   uint32_t t = e;
   while (t < send) {
-    if ((bm[t >> 5] >> (t & 31u)) & 1u) return t | 0x80000000u;
-    t = nextpos(m, inb, n, lencap, t);
+    const uint32_t w = mk[t >> 5], v = m[t], v2 = t + 1 < n ? m[t + 1] : 0u;
+    if ((w >> (t & 31u)) & 1u) return t | 0x80000000u;
+    t = nextpos_v(m, inb, n, lencap, t, lazy_of(v, v2, lencap));
   }
   return t;
 }
 
 // Each of the first `walkers` threads (a power of two, 32 <= B / walkers <= 1024) walks its segment of B / walkers positions
-// from the segment's start, marking token starts in an LDS bitmap; the join (below) puts the true chain in their
-// place: where the chain meets a walk the two coincide from there on (the chain is a pure function of the position),
-// so only the walk's prefix is wrong; where it never meets one, the walk is wrong whole.
-// mark[b] receives the bitmap of the chain (B bits).
+// from the segment's start, marking token starts in the block's chain bitmap (B bits, in LDS when LM, else in
+// mark[b] itself; each walker writes its own segment's words); the join (below) puts the true chain in their place:
+// where the chain meets a walk the two coincide from there on (the chain is a pure function of the position), so
+// only the walk's prefix is wrong; where it never meets one, the walk is wrong whole. The workgroup's barriers order
+// one thread's bitmap stores before another's loads. mark[b] receives the bitmap of the chain.
+template <bool LM>
 __device__ __forceinline__ void enc_parse_block(const Block* __restrict__ blocks, const uint8_t* __restrict__ in, uint32_t blog,
                                        uint32_t walkers, uint32_t lencap, uint32_t* __restrict__ match,
                                        uint32_t* __restrict__ mark, uint32_t b) {
-  extern __shared__ uint32_t sm[];  // bitmap (B/32 words), ex, sin, sout (walkers words each), windows (16 each)
-  const uint32_t B = 1u << blog, SEG = B / walkers, T = blockDim.x;  // (T >= walkers)
+  extern __shared__ uint32_t sm[];  // (LM: the bitmap, B/32 words), ex, sin, sout (walkers words each), windows
+  const uint32_t B = 1u << blog, SEG = B / walkers;  // (blockDim.x >= walkers)
   const uint32_t k = threadIdx.x;
-  uint32_t* bm = sm;
-  uint32_t* ex = sm + B / 32u;
+  uint32_t* ex = LM ? sm + B / 32u : sm;
   uint32_t* sin = ex + walkers;    // a segment's entry as walked
   uint32_t* sout = sin + walkers;  // where that walk met walk q (| 1 << 31), or left the segment
   const Block blk = blocks[b];
   const uint8_t* inb = in + blk.src;
   const uint32_t n = enc_len(blk, blog);
   uint32_t* m = match + ((size_t)b << blog);
+  uint32_t* const out = mark + ((size_t)b << (blog - 5));
+  uint32_t* mk = LM ? sm : out;  // the bitmap being gathered
   if (blk.level) { uint32_t nc; level_params(blk.level, nc, lencap); }
-  for (uint32_t i = k; i < B / 32u; i += T) bm[i] = 0;
-  __syncthreads();
   if (k < walkers) {
-    // m is read 16 entries (64 bytes) at a time into this walker's LDS window: the many walks in flight would
-    // otherwise fetch a line of m for each step, their lines leaving the caches before the next step comes
-    // A step goes from p straight to the window's next position with a match (the positions before it are
-    // literals, one chain position each) or, with none, to the window's end.
-    uint32_t* win = sout + walkers + 16 * k;
+    // m is read PARSE_WIN entries at a time into this walker's LDS window: the many walks in flight would otherwise
+    // fetch a line of m for each step, their lines leaving the caches before the next step comes. A step goes from p straight to the window's next position with a match
+    // (the positions before it are literals, one chain position each) or, with none, to the window's end.
+    uint32_t* win = sout + walkers + (PARSE_WIN + 1u) * k;
     const uint32_t s = k * SEG, send = min(s + SEG, n);
-    uint32_t p = s, wb = s - 16u, mm = 0;  // window base (none yet), which of its positions have a match
+    MarkWriter mw{mk, s >> 5, 0u, SEG >= 128u ? 4u : SEG >= 64u ? 2u : 1u};
+    uint32_t p = s, wb = s - PARSE_WIN, mm = 0;  // window base (none yet), which of its positions have a match
     while (p < send) {
-      if (p - wb >= 16u) {
-        wb = p & ~15u;
-        uint32_t v[17];
-        __builtin_memcpy(v, m + wb, 64);
-        v[16] = m[min(wb + 16u, B - 1u)];  // (used only when wb + 16 < n)
+      if (p - wb >= PARSE_WIN) {
+        wb = p & ~(PARSE_WIN - 1u);
+        uint32_t v[PARSE_WIN + 1];
+        __builtin_memcpy(v, m + wb, 4 * PARSE_WIN);
+        v[PARSE_WIN] = m[min(wb + PARSE_WIN, B - 1u)];  // (used only when wb + PARSE_WIN < n)
         mm = 0;
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
+        for (uint32_t q = 0; q < PARSE_WIN; ++q) {
           const uint32_t lz = lazy_of(v[q], wb + q + 1u < n ? v[q + 1] : 0u, lencap);
           win[q] = lz;
           mm |= (match_at(lz, wb + q, n) ? 1u : 0u) << q;
         }
       }
       const uint32_t ahead = mm & (~0u << (p - wb));
-      const uint32_t q = ahead ? wb + (uint32_t)__builtin_ctz(ahead) : wb + 16u;
+      const uint32_t q = ahead ? wb + (uint32_t)__builtin_ctz(ahead) : wb + PARSE_WIN;
       if (q >= send) {  // literals to the segment's end
-        mark_range(bm, p, send);
+        mw.range(p, send);
         p = send;
       } else if (!ahead) {
-        mark_range(bm, p, q);
+        mw.range(p, q);
         p = q;
       } else {
-        mark_range(bm, p, q + 1);  // (a segment owns whole words: SEG is a multiple of 32)
+        mw.range(p, q + 1);  // (a segment owns whole words: SEG is a multiple of 32)
         p = nextpos_v(m, inb, n, lencap, q, win[q - wb]);
       }
     }
+    mw.to(s + SEG);  // the rest of the segment's words (zero past the block's end)
     ex[k] = s < n ? p : s;
   }
   __syncthreads();
@@ -951,7 +989,7 @@ __device__ __forceinline__ void enc_parse_block(const Block* __restrict__ blocks
   if (k < nseg) {
     const uint32_t e = k ? ex[k - 1] : 0u;
     sin[k] = e;
-    sout[k] = join_walk(m, inb, n, lencap, bm, e, min(k * SEG + SEG, n));
+    sout[k] = join_walk(m, inb, n, lencap, mk, e, min(k * SEG + SEG, n));
   }
   __syncthreads();
   if (k == 0) {
@@ -959,7 +997,7 @@ __device__ __forceinline__ void enc_parse_block(const Block* __restrict__ blocks
     for (uint32_t q = 0; q < nseg; ++q) {
       uint32_t r = sout[q];
       if (sin[q] != e) {
-        r = join_walk(m, inb, n, lencap, bm, e, min(q * SEG + SEG, n));
+        r = join_walk(m, inb, n, lencap, mk, e, min(q * SEG + SEG, n));
         sin[q] = e;
         sout[q] = r;
       }
@@ -968,22 +1006,30 @@ __device__ __forceinline__ void enc_parse_block(const Block* __restrict__ blocks
   }
   __syncthreads();
   if (k < nseg) {
+    // the chain's positions replace the walk's marks from the segment's start up to the meeting point: whole words
+    // stored anew, the word holding the meeting point merged with the walk's marks from there on
     const uint32_t s = k * SEG, send = min(s + SEG, n), r = sout[k];
     const uint32_t clear_to = r >> 31 ? r & 0x7FFFFFFFu : send;
-    for (uint32_t x = s; x < clear_to;) {
-      if ((x & 31u) == 0 && x + 32 <= clear_to) { bm[x >> 5] = 0; x += 32; }
-      else { bm[x >> 5] &= ~(1u << (x & 31u)); ++x; }
+    MarkWriter mw{mk, s >> 5, 0u, 1u};
+    for (uint32_t p = sin[k]; p < clear_to; p = nextpos(m, inb, n, lencap, p)) {
+      mw.to(p);
+      mw.bits |= 1u << (p & 31u);
     }
-    for (uint32_t p = sin[k]; p < clear_to; p = nextpos(m, inb, n, lencap, p)) bm[p >> 5] |= 1u << (p & 31u);
+    mw.to(clear_to);
+    if (clear_to & 31u) mk[mw.cw] = mw.bits | (mk[mw.cw] & (~0u << (clear_to & 31u)));
   }
-  __syncthreads();
-  uint32_t* mk = mark + ((size_t)b << (blog - 5));
-  for (uint32_t i = k; i < B / 32u; i += T) mk[i] = bm[i];
+  if (LM) {
+    __syncthreads();
+    for (uint32_t i = k; i < B / 128u; i += blockDim.x)
+      reinterpret_cast<uint4*>(out)[i] = reinterpret_cast<const uint4*>(mk)[i];
+    __syncthreads();  // (the bitmap's LDS is the next block's, in the batched form)
+  }
 }
+template <bool LM>
 __global__ __launch_bounds__(1024) void k_enc_parse(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
                                                     uint32_t blog, uint32_t walkers, uint32_t lencap,
                                                     uint32_t* __restrict__ match, uint32_t* __restrict__ mark) {
-  enc_parse_block(blocks, in, blog, walkers, lencap, match, mark, blockIdx.x);
+  enc_parse_block<LM>(blocks, in, blog, walkers, lencap, match, mark, blockIdx.x);
 }
 
 // ---------------------------------------------------------------- 6. tokens and streams (one wavefront per block)
