@@ -179,7 +179,7 @@ struct EncShape {
   uint32_t blog, hbits, keybits, walkers;
 };
 // the parse's segments per block: one per PARSE_SPAN positions but at least PARSE_WALKERS_MIN, each at least 32
-// positions, and the parse's LDS (the chain bitmap, 3 words and a 16-word window per walker) within 160 KiB.
+// positions, and the parse's LDS (3 words and a window per walker) within 160 KiB.
 // Fewer, longer segments cost more serial walking; more, shorter ones cost more joins (a long match crosses many).
 uint32_t walkers_of(uint32_t blog) {
   uint32_t w = std::min<uint32_t>(std::max<uint32_t>(PARSE_WALKERS_MIN, (1u << blog) / PARSE_SPAN), (1u << blog) / 32);
@@ -302,10 +302,8 @@ int zgpu_open(int ordinal, zgpu_t** out) {
                             160 * 1024);
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_enc_sort_l), hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)sort_l_lds_bytes(20));
-  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_enc_parse<true>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_enc_parse<false>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_enc_parse), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            160 * 1024);
   if (const char* pe = getenv("ZERO_PROFILE")) g->profile = atoi(pe) != 0;
   *out = g;
   return ZGPU_OK;
@@ -463,12 +461,8 @@ static int encode_run(zgpu_t* g, const uint8_t* d_in, const Block* blocks, uint3
   k_enc_match<<<posgroups, 256, 0, st>>>(blocks, d_in, blog, ncand, lencap, srt, mt);
   if (prof) stage(g, st, "match");
   const size_t parse_lds = parse_lds_words(blog, S.walkers) * 4;
-  if (parse_lds_marks(blog))
-    k_enc_parse<true><<<nb, std::max(64u, S.walkers), parse_lds, st>>>(blocks, d_in, blog, S.walkers, lencap, mt,
-                                                                      (uint32_t*)(base + L.mark));
-  else
-    k_enc_parse<false><<<nb, std::max(64u, S.walkers), parse_lds, st>>>(blocks, d_in, blog, S.walkers, lencap, mt,
-                                                                       (uint32_t*)(base + L.mark));
+  k_enc_parse<<<nb, std::max(64u, S.walkers), parse_lds, st>>>(blocks, d_in, blog, S.walkers, lencap, mt,
+                                                               (uint32_t*)(base + L.mark));
   if (prof) stage(g, st, "parse");
   k_enc_tokens<<<nb, 64, 0, st>>>(blocks, d_in, blog, lencap, mt, (uint32_t*)(base + L.mark), T);
   if (prof) stage(g, st, "tokens");

@@ -27,14 +27,13 @@ __host__ __device__ inline uint32_t hbits_of(uint32_t blog) { return blog - 3u <
 // the parse's window of matches per walker (entries; 32, a whole 128-byte line, was slower on pages with long
 // matches, whose walks use few entries of a window)
 constexpr uint32_t PARSE_WIN = 16;
-// where the parse gathers a block's chain bitmap: in LDS (copied out at the end) for blocks up to 2^16 bytes, whose
-// bitmap is small beside the rest of the parse's LDS; else in the block's mark words in HBM, each walker storing its
-// own segment's words, so that the bitmap (32 KiB at 2^18, 128 KiB at 2^20) does not limit the workgroups per CU
-__host__ __device__ inline bool parse_lds_marks(uint32_t blog) { return blog <= 16u; }
-// the parse's LDS words for `walkers` segments: the bitmap when it is kept in LDS, then per walker its exit, entry,
-// meeting point and a window (PARSE_WIN + 1 words apart, so that the walkers' window accesses spread over the banks)
+// the parse's LDS words for `walkers` segments: per walker its exit, entry, meeting point and a window (PARSE_WIN + 1
+// words apart, so that the walkers' window accesses spread over the banks). The chain bitmap is gathered in the
+// block's mark words in HBM, each walker storing its own segment's words, so that it (32 KiB at 2^18, 128 KiB at
+// 2^20) does not limit the workgroups per CU.
 __host__ __device__ inline size_t parse_lds_words(uint32_t blog, uint32_t walkers) {
-  return (parse_lds_marks(blog) ? ((size_t)1 << blog) / 32 : 0) + (4 + PARSE_WIN) * (size_t)walkers;
+  (void)blog;
+  return (4 + PARSE_WIN) * (size_t)walkers;
 }
 constexpr uint32_t PARSE_SPAN = 1024;       // default parse segment: positions per walker
 constexpr uint32_t PARSE_WALKERS_MIN = 128;  // default walkers per block, at least (each segment still >= 32 positions)
@@ -891,9 +890,18 @@ struct MarkWriter {
     bits = 0;
     ++cw;
   }
-  // stores the words before position a's
+  // stores the words before position a's: the gathered word and the rest of its group, then whole groups of zero
+  // words (a long match skips many), then the zero words before a's in its group
   __device__ void to(uint32_t a) {
-    while ((a >> 5) > cw) next();
+    const uint32_t t = a >> 5;
+    if (t <= cw) return;
+    do next(); while (cw < t && (cw & (g - 1u)));
+    for (; cw + g <= t; cw += g) {
+      if (g == 4u) *reinterpret_cast<uint4*>(mk + cw) = make_uint4(0u, 0u, 0u, 0u);
+      else if (g == 2u) *reinterpret_cast<uint2*>(mk + cw) = make_uint2(0u, 0u);
+      else mk[cw] = 0u;
+    }
+    while (cw < t) next();
   }
   // sets the bits a .. b - 1 (a < b <= a + 32, a at or past the gathered word's start)
   __device__ void range(uint32_t a, uint32_t b) {
@@ -918,27 +926,25 @@ __device__ inline uint32_t join_walk(uint32_t* m, const uint8_t* inb, uint32_t n
 }
 
 // Each of the first `walkers` threads (a power of two, 32 <= B / walkers <= 1024) walks its segment of B / walkers positions
-// from the segment's start, marking token starts in the block's chain bitmap (B bits, in LDS when LM, else in
-// mark[b] itself; each walker writes its own segment's words); the join (below) puts the true chain in their place:
-// where the chain meets a walk the two coincide from there on (the chain is a pure function of the position), so
-// only the walk's prefix is wrong; where it never meets one, the walk is wrong whole. The workgroup's barriers order
-// one thread's bitmap stores before another's loads. mark[b] receives the bitmap of the chain.
-template <bool LM>
+// from the segment's start, marking token starts in the block's chain bitmap mark[b] (B bits; each walker writes its
+// own segment's words); the join (below) puts the true chain in their place: where the chain meets a walk the two
+// coincide from there on (the chain is a pure function of the position), so only the walk's prefix is wrong; where it
+// never meets one, the walk is wrong whole. The workgroup's barriers order one thread's bitmap stores before another's
+// loads.
 __device__ __forceinline__ void enc_parse_block(const Block* __restrict__ blocks, const uint8_t* __restrict__ in, uint32_t blog,
                                        uint32_t walkers, uint32_t lencap, uint32_t* __restrict__ match,
                                        uint32_t* __restrict__ mark, uint32_t b) {
-  extern __shared__ uint32_t sm[];  // (LM: the bitmap, B/32 words), ex, sin, sout (walkers words each), windows
+  extern __shared__ uint32_t sm[];  // ex, sin, sout (walkers words each), windows
   const uint32_t B = 1u << blog, SEG = B / walkers;  // (blockDim.x >= walkers)
   const uint32_t k = threadIdx.x;
-  uint32_t* ex = LM ? sm + B / 32u : sm;
+  uint32_t* ex = sm;
   uint32_t* sin = ex + walkers;    // a segment's entry as walked
   uint32_t* sout = sin + walkers;  // where that walk met walk q (| 1 << 31), or left the segment
   const Block blk = blocks[b];
   const uint8_t* inb = in + blk.src;
   const uint32_t n = enc_len(blk, blog);
   uint32_t* m = match + ((size_t)b << blog);
-  uint32_t* const out = mark + ((size_t)b << (blog - 5));
-  uint32_t* mk = LM ? sm : out;  // the bitmap being gathered
+  uint32_t* mk = mark + ((size_t)b << (blog - 5));
   if (blk.level) { uint32_t nc; level_params(blk.level, nc, lencap); }
   if (k < walkers) {
     // m is read PARSE_WIN entries at a time into this walker's LDS window: the many walks in flight would otherwise
@@ -1018,18 +1024,11 @@ __device__ __forceinline__ void enc_parse_block(const Block* __restrict__ blocks
     mw.to(clear_to);
     if (clear_to & 31u) mk[mw.cw] = mw.bits | (mk[mw.cw] & (~0u << (clear_to & 31u)));
   }
-  if (LM) {
-    __syncthreads();
-    for (uint32_t i = k; i < B / 128u; i += blockDim.x)
-      reinterpret_cast<uint4*>(out)[i] = reinterpret_cast<const uint4*>(mk)[i];
-    __syncthreads();  // (the bitmap's LDS is the next block's, in the batched form)
-  }
 }
-template <bool LM>
 __global__ __launch_bounds__(1024) void k_enc_parse(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
                                                     uint32_t blog, uint32_t walkers, uint32_t lencap,
                                                     uint32_t* __restrict__ match, uint32_t* __restrict__ mark) {
-  enc_parse_block<LM>(blocks, in, blog, walkers, lencap, match, mark, blockIdx.x);
+  enc_parse_block(blocks, in, blog, walkers, lencap, match, mark, blockIdx.x);
 }
 
 // ---------------------------------------------------------------- 6. tokens and streams (one wavefront per block)

@@ -170,10 +170,7 @@ __global__ __launch_bounds__(1024) void k_plan_parse(const PTab* __restrict__ t,
                                                      const uint8_t* in, uint32_t lencap) {
   PLAN_LOOP(P_PARSE, [](const PSeg&) { return 1u; }, 1u) {
     const PSeg& g = t->s[c];
-    if (parse_lds_marks(g.blog))
-      enc_parse_block<true>(enc + g.first, in, g.blog, g.walkers, lencap, g.mt, g.mark, bl);
-    else
-      enc_parse_block<false>(enc + g.first, in, g.blog, g.walkers, lencap, g.mt, g.mark, bl);
+    enc_parse_block(enc + g.first, in, g.blog, g.walkers, lencap, g.mt, g.mark, bl);
   }
 }
 __global__ __launch_bounds__(64) void k_plan_tokens(const PTab* __restrict__ t, uint32_t* ctr, const Block* enc,
