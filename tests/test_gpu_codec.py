@@ -270,6 +270,29 @@ def test_encode_large_buckets(gpu, bl):
         assert gpu.decompress([(m, len(b), p) for b, (m, p) in zip(blocks, got)], bl) == data
 
 
+@pytest.mark.parametrize("bl", [18, 20])
+def test_encode_buckets_past_16_bits(gpu, bl):
+    """the hash kernel's bucket counters are 16-bit halves of LDS words, kept exact past 65535 by a log of the
+    atomics that carried: two runs of one byte value, each far longer than 65536 positions, one counted in the low half
+    of its word and one in the high half (so both a low half's carry and a whole word's wrap occur, several times at
+    2^20); payloads equal the oracle's model and decode back"""
+    hb = bl - 3 if bl - 3 < 15 else 15
+    hz = _hash5(0, 0) >> (32 - hb)
+    v = next(v for v in range(1, 256) if (_hash5(v * 0x01010101, v) >> (32 - hb)) & 1 != hz & 1)
+    B = 1 << bl
+    run = 100_000 if bl == 18 else 450_000
+    rng = np.random.default_rng(7 + bl)
+    d = rng.integers(0, 256, B, dtype=np.uint8)
+    d[100:100 + run] = 0
+    d[200 + run:200 + 2 * run] = v
+    blk = d.tobytes()
+    for level in (1, 5):
+        (method, payload), = gpu.compress(blk, bl, level)
+        want, st = O.gpu_compress_block(blk, bl, level)
+        assert st == 0 and method == 1 and payload == want, level
+        assert gpu.decompress([(method, B, payload)], bl) == blk
+
+
 def test_encode_past_one_launch(gpu):
     """more than 2^31 bytes in one zgpu_encode_blocks call (8193 blocks of 2^18) runs as several launches and gives
     every block the payload it gets alone: the blocks are 128 copies of 64 blocks, so each must match its copy's

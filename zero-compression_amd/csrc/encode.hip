@@ -131,28 +131,90 @@ __device__ inline uint32_t lds_inc(uint32_t* a, uint32_t i, bool on, bool crowd)
   if (same) return b + mbcnt64(sm);
   return on ? atomicAdd(&a[i], 1u) : 0u;
 }
-// k_enc_hash's LDS: the bucket counters (at least 4096 words: later the 16 wavefronts' staging), 16 wavefront sums,
-// a flag, the partition cursors, and per wavefront its partition counts and bases
+// k_enc_hash's bucket counters, two 16-bit counters to a word (64 KiB for 2^15 buckets: two workgroups per CU). A
+// count past 65535 is kept exact by a log of the atomics that carried: one whose low half passed a multiple of 65536
+// (the word's index) and one whose word passed 2^32 (the index | 1 << 31). With adds of at most 64 each carry is one
+// crossing, and a block of 2^20 positions makes at most 16 of each kind. The true counts of word w: low =
+// (w & 0xFFFF) + 65536 x (its low carries), high = (2^32 x (its word carries) + w - low) >> 16.
+constexpr uint32_t HASH_CARRY_MAX = 32;
+__device__ inline void lds_count16(uint32_t* a, uint32_t* carries, uint32_t i, bool on, bool crowd) {
+  uint32_t w = i >> 1, add = 1u << ((i & 1u) << 4), old = 0;
+  bool did = false;
+  if (crowd) {  // (uniform) the lanes on the first lane's counter as one atomic, the others one each
+    const uint64_t act = ballot(on);
+    if (!act) return;
+    const uint32_t lead = (uint32_t)__builtin_ctzll(act);
+    const uint32_t first = readlane(i, lead);
+    const bool same = on && i == first;
+    const uint32_t k = popc64(ballot(same));
+    if (lane_id() == lead) {
+      add = k << ((first & 1u) << 4);
+      old = atomicAdd(&a[w], add);
+      did = true;
+    } else if (on && !same) {
+      old = atomicAdd(&a[w], add);
+      did = true;
+    }
+  } else if (on) {
+    old = atomicAdd(&a[w], add);
+    did = true;
+  }
+  if (did) {
+    if ((old & 0xFFFFu) + (add & 0xFFFFu) > 0xFFFFu) {
+      const uint32_t e = atomicAdd(&carries[0], 1u);
+      if (e < HASH_CARRY_MAX) carries[1 + e] = w;
+    }
+    if ((uint64_t)old + add > 0xFFFFFFFFull) {
+      const uint32_t e = atomicAdd(&carries[0], 1u);
+      if (e < HASH_CARRY_MAX) carries[1 + e] = w | 0x80000000u;
+    }
+  }
+}
+// the true count of counter i (0 where !in), the wavefront's lanes taking the 64 consecutive counters from i - lane
+// (all lanes active); ev: lane e's entry of the carry log, ~0 past its end. Only the entries on the step's words are
+// looked at.
+__device__ inline uint32_t count16_at(const uint32_t* a, uint32_t ev, uint32_t i, bool in) {
+  const uint32_t w = in ? a[i >> 1] : 0u;
+  const uint32_t base = (i - lane_id()) >> 1;
+  uint64_t hit = ballot(((ev & 0x7FFFFFFFu) - base) <= 32u);  // (words base .. base + 32: i - lane may be odd)
+  uint32_t lc = 0, wc = 0;
+  for (; hit; hit &= hit - 1) {
+    const uint32_t v = readlane(ev, (uint32_t)__builtin_ctzll(hit));
+    if ((v & 0x7FFFFFFFu) == (i >> 1)) {
+      if (v >> 31) ++wc;
+      else ++lc;
+    }
+  }
+  if (!in) return 0u;
+  const uint32_t lo = (w & 0xFFFFu) + (lc << 16);
+  return (i & 1u) ? (uint32_t)((((uint64_t)wc << 32) + w - lo) >> 16) : lo;
+}
+// k_enc_hash's LDS: the bucket counters (whose words, once scanned, hold the 16 wavefronts' staging and then their
+// partition counts and bases), 16 wavefront sums, a flag, the partition cursors and the carry log
 constexpr uint32_t HASH_STAGE = 256;  // positions a wavefront stages per step
-__host__ __device__ inline uint32_t hash_counter_words(uint32_t hbits) {
-  return (1u << hbits) > 16u * HASH_STAGE ? 1u << hbits : 16u * HASH_STAGE;
+__host__ __device__ inline uint32_t hash_counter_words(uint32_t hbits, uint32_t pbits) {
+  const uint32_t later = 16u * HASH_STAGE + 32u * (1u << pbits);
+  return (1u << hbits) / 2u > later ? (1u << hbits) / 2u : later;
 }
 __host__ __device__ inline size_t hash_lds_words(uint32_t hbits, uint32_t pbits) {
-  return (size_t)hash_counter_words(hbits) + 17 + 33 * ((size_t)1 << pbits);
+  return (size_t)hash_counter_words(hbits, pbits) + 17 + ((size_t)1 << pbits) + 1 + HASH_CARRY_MAX;
 }
 __device__ __forceinline__ void enc_hash_block(const Block* __restrict__ blocks, const uint8_t* __restrict__ in, uint32_t blog,
                                       uint32_t hbits, uint32_t pbits, uint32_t* __restrict__ tmp,
                                       uint32_t* __restrict__ bstart, uint32_t b) {
   extern __shared__ uint32_t cnt[];  // hash_lds_words(hbits, pbits)
   const uint32_t tid = threadIdx.x;
-  const uint32_t HBITS = hbits, HB = 1u << HBITS, CW = hash_counter_words(hbits);
+  const uint32_t HBITS = hbits, HB = 1u << HBITS, CW = hash_counter_words(hbits, pbits);
   const Block blk = blocks[b];
   const uint8_t* inb = in + blk.src;
   const uint32_t n = enc_len(blk, blog);
   const uint32_t count = n >= 4 ? n - 3 : 0;
   uint32_t* bigb = cnt + CW + 16;  // (after the 16 wavefront sums) a bucket of more than 4096 entries exists
-  for (uint32_t i = tid; i < HB; i += 1024) cnt[i] = 0;
-  if (tid == 0) *bigb = 0u;
+  const uint32_t NP = 1u << pbits, pmask = (1u << blog) - 1u;
+  uint32_t* pcur = cnt + CW + 17;                   // [NP] next free slot of each partition
+  uint32_t* carries = cnt + CW + 17 + NP;           // [1 + HASH_CARRY_MAX] the counters' carry log
+  for (uint32_t i = tid; i < HB / 2u; i += 1024) cnt[i] = 0;
+  if (tid == 0) { *bigb = 0u; carries[0] = 0u; }
   __syncthreads();
   // 4 positions per lane from one 8-byte load, the next load in flight while these are counted
   {
@@ -161,17 +223,19 @@ __device__ __forceinline__ void enc_hash_block(const Block* __restrict__ blocks,
       const uint64_t nx = n >= 8 ? ld8z(inb, n, q + 4096) : 0ull;
       const bool crowd = crowded(hash5_of(x, 0) >> (32 - HBITS));
 #pragma unroll
-      for (uint32_t j = 0; j < 4; ++j) lds_inc(cnt, hash5_of(x, j) >> (32 - HBITS), q + j < count, crowd);
+      for (uint32_t j = 0; j < 4; ++j) lds_count16(cnt, carries, hash5_of(x, j) >> (32 - HBITS), q + j < count, crowd);
       x = nx;
     }
   }
   __syncthreads();
-  // exclusive scan of the HB counters, HB/16 per wavefront, 64 consecutive counters per step (no LDS bank conflicts)
+  // exclusive scan of the HB counters, HB/16 per wavefront, 64 consecutive counters per step; the bucket starts go to
+  // bstart, the partitions' first ones to their cursors
   const uint32_t wv = tid >> 6, lane = tid & 63u, chunk = HB / 16u;
+  const uint32_t ev = lane < min(carries[0], HASH_CARRY_MAX) ? carries[1 + lane] : ~0u;
   uint32_t* wsum = cnt + CW;  // [16]
   {
     uint32_t s = 0;
-    for (uint32_t i = lane; i < chunk; i += 64) s += cnt[wv * chunk + i];
+    for (uint32_t i0 = 0; i0 < chunk; i0 += 64) s += count16_at(cnt, ev, wv * chunk + i0 + lane, i0 + lane < chunk);
     s = wave_sum(s);
     if (lane == 0) wsum[wv] = s;
   }
@@ -183,9 +247,12 @@ __device__ __forceinline__ void enc_hash_block(const Block* __restrict__ blocks,
     for (uint32_t i0 = 0; i0 < chunk; i0 += 64) {
       const uint32_t i = wv * chunk + i0 + lane;
       const bool in = i0 + lane < chunk;
-      const uint32_t c = in ? cnt[i] : 0u;
+      const uint32_t c = count16_at(cnt, ev, i, in);
       const uint32_t inc = wave_scan_incl(c);
-      if (in) { cnt[i] = run + inc - c; bs[i] = run + inc - c; }
+      if (in) {
+        bs[i] = run + inc - c;
+        if ((i & ((1u << (HBITS - pbits)) - 1u)) == 0) pcur[i >> (HBITS - pbits)] = run + inc - c;
+      }
       if (in && c > 4096u) *bigb = 1u;  // (lanes that write write the same value)
       run += __builtin_amdgcn_readlane(inc, 63);
     }
@@ -199,13 +266,10 @@ __device__ __forceinline__ void enc_hash_block(const Block* __restrict__ blocks,
   // the wavefront's staging, go out as runs of consecutive slots (a store instruction covers a few runs, instead of
   // about one slot in every partition: each slot written at a scattered place costs an L2 request of its own). No
   // workgroup barrier inside the loop.
-  const uint32_t NP = 1u << pbits, pmask = (1u << blog) - 1u;
-  uint32_t* pcur = cnt + CW + 17;            // [NP] next free slot of each partition
-  uint32_t* wc = pcur + NP + wv * 2u * NP;   // [16][2 NP]: per partition this step's count, then its offset among
-                                             // the step's positions; [NP + i]: its first slot minus that offset
+  uint32_t* wc = cnt + 16u * HASH_STAGE + wv * 2u * NP;  // [16][2 NP] (counter words): per partition this step's
+                                                         // count, then its offset among the step's positions;
+                                                         // [NP + i]: its first slot minus that offset
   uint32_t* wst = cnt + wv * HASH_STAGE;     // [16][HASH_STAGE] (the counters' words, no longer needed)
-  if (tid < NP) pcur[tid] = cnt[tid << (HBITS - pbits)];
-  __syncthreads();
   uint32_t* tb = tmp + ((size_t)b << blog);
   uint64_t x = n >= 8 ? ld8z(inb, n, 4 * tid) : ld8z_small(inb, n, 4 * tid);
   for (uint32_t q = 4 * tid; q - 4 * lane < count; q += 4096) {  // (uniform: a wavefront runs while any lane has work)
