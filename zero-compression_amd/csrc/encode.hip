@@ -1089,9 +1089,10 @@ __device__ __forceinline__ void enc_parse_block(const Block* __restrict__ blocks
     if (clear_to & 31u) mk[mw.cw] = mw.bits | (mk[mw.cw] & (~0u << (clear_to & 31u)));
   }
 }
-__global__ __launch_bounds__(1024) void k_enc_parse(const Block* __restrict__ blocks, const uint8_t* __restrict__ in,
-                                                    uint32_t blog, uint32_t walkers, uint32_t lencap,
-                                                    uint32_t* __restrict__ match, uint32_t* __restrict__ mark) {
+// at most 64 VGPRs (8 wavefronts per SIMD): two workgroups of 1024 walkers (2^20-byte blocks) then fit a CU, not one
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void k_enc_parse(
+    const Block* __restrict__ blocks, const uint8_t* __restrict__ in, uint32_t blog, uint32_t walkers,
+    uint32_t lencap, uint32_t* __restrict__ match, uint32_t* __restrict__ mark) {
   enc_parse_block(blocks, in, blog, walkers, lencap, match, mark, blockIdx.x);
 }
 
