@@ -4,11 +4,13 @@
 //
 // Per launch: nb blocks, block b's raw bytes at in + blocks[b].src (blocks[b].raw_len of them). Per-block scratch
 // is indexed by b (EncLayout in capi.hip). Stages:
-//   k_enc_hash      hash5 of every position, bucket counts (in LDS) and bstart, the positions grouped by partition
+//   k_enc_hash      hash5 of every position, bucket counts (16-bit, in LDS) and bstart, the positions grouped by
+//                   partition
 //   k_enc_place     a partition's positions into their buckets, through LDS
 //   k_enc_sort_s    buckets of <= 16 entries sorted in registers, <= 64 across a wavefront; larger ones queued
 //   k_enc_sort_m    queued buckets of 65..4096 entries: bitonic sort in LDS
-//   k_enc_sort_l    larger buckets: a workgroup's stable LSD radix sort through global memory
+//   k_enc_sort_l    larger buckets: the dominant key's entries through a bitmap of positions in LDS, the few others
+//                   ranked among themselves; with many others, a stable LSD radix sort through global memory
 //   k_enc_match     the best match of every position among its run check and sorted neighbours
 //   k_enc_parse     the greedy chain (with lazy matching): segment walkers in parallel, then a join of their walks
 //   k_enc_tokens    the chain -> literals and tokens; groups, recent offsets, tok/ext/sym/groups, offset bits
